@@ -1,0 +1,553 @@
+// extern "C" boundary of libduct_hip.so (declared in include/duct_hip.h).
+//
+// Library state: one device, one stream, the device copies of the Dirichlet
+// and Neumann eigenbases (uploaded once by duct_setup_), and two planar
+// scratch buffers of nplanes * NP^2 doubles that the projection passes
+// ping-pong through.  Host-array entry points (duct_helm3d_, duct_int2d3d_,
+// duct_solvediag_) additionally keep device copies of their fields.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/duct_hip.h"
+#include "common.h"
+#include "proj.h"
+
+using namespace duct;
+
+namespace {
+
+struct DevBasis {
+    bool ready = false;
+    int m = 0, n = 0, NP = 0;
+    double* Afwd = nullptr;  // P^-1, fragment layout (passes 1, 2)
+    double* Abwd = nullptr;  // P,    fragment layout (passes 3, 4)
+    double* lam = nullptr;   // NP, zero padded
+    double* DC = nullptr;    // m x m
+    double* AM1 = nullptr;   // 4 x 4
+    double* D2b = nullptr;   // m x 2: D2(:,1), D2(:,m)
+    double* prow = nullptr;  // P(iofzero, :), NP
+    double pdiag = 0.0, wanted = 0.0;
+    int gauge = -1;          // 0-based eigen-index of the null mode (Neumann)
+};
+
+struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+};
+
+enum ProfKind { PK_C2P = 0, PK_P2P_DIV = 1, PK_GAUGE = 2, PK_P2P = 3, PK_P2C = 4, PK_BOUNDARY = 5, PK_BCCORR = 6 };
+
+struct Lib {
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBasis basis[2];  // 0 = 'd', 1 = 'n'
+    double* plan[2] = {nullptr, nullptr};
+    size_t plan_bytes = 0;
+    d2v* hfield[2] = {nullptr, nullptr};  // device fields for host-array entries
+    size_t hfield_bytes = 0;
+    bool prof = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> pool;
+    std::string err;
+};
+
+Lib L;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    L.err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                        __FILE__, __LINE__);                                              \
+    } while (0)
+
+int ensure_init()
+{
+    if (L.init) return OK;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    L.device = dev;
+    L.init = true;
+    return OK;
+}
+
+template <class T>
+int ensure_buf(T*& p, size_t& cap, size_t bytes)
+{
+    if (bytes <= cap && p) return OK;
+    if (p) {
+        HIPCHK(hipStreamSynchronize(L.stream));
+        HIPCHK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+    }
+    hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    cap = bytes;
+    return OK;
+}
+
+int ensure_planar(size_t bytes)
+{
+    if (bytes <= L.plan_bytes && L.plan[0]) return OK;
+    size_t c0 = L.plan_bytes, c1 = L.plan_bytes;
+    int st = ensure_buf(L.plan[0], c0, bytes);
+    if (st) return st;
+    st = ensure_buf(L.plan[1], c1, bytes);
+    if (st) return st;
+    L.plan_bytes = bytes;
+    return OK;
+}
+
+hipEvent_t take_event()
+{
+    if (!L.pool.empty()) {
+        hipEvent_t e = L.pool.back();
+        L.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Bracket one launch with events when timing is enabled.
+struct Timed {
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit Timed(int k) : kind(k)
+    {
+        if (!L.prof) return;
+        a = take_event();
+        b = take_event();
+        if (a) (void)hipEventRecord(a, L.stream);
+    }
+    ~Timed()
+    {
+        if (!L.prof || !a || !b) return;
+        (void)hipEventRecord(b, L.stream);
+        L.recs.push_back({kind, a, b});
+    }
+};
+
+int bound_index(char c)
+{
+    if (c == 'd') return 0;
+    if (c == 'n') return 1;
+    return -1;
+}
+
+// The four projection passes, the gauge and the boundary for one group of
+// fields sharing a basis.  bmode is a BoundaryMode.
+int run_group(const FieldGroup& g, int bidx, int bmode)
+{
+    DevBasis& B = L.basis[bidx];
+    if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
+    if (g.m != B.m) return fail(ERR_ARG, "m=%d but the '%c' basis was set up for m=%d", g.m, bidx ? 'n' : 'd', B.m);
+    const int NP = B.NP;
+    const long cplx_cols = (long)g.nf * g.nmodes * NP;
+    const long planes = (long)g.nf * g.nmodes * 2;
+    int st = ensure_planar((size_t)planes * NP * NP * sizeof(double));
+    if (st) return st;
+
+    ProjArgs a{};
+    a.g = g;
+    a.lam = B.lam;
+    a.gauge = bidx == 1 ? B.gauge : -1;
+
+    {   // pass 1: T = (P^-1 F)^T
+        Timed t(PK_C2P);
+        a.Afrag = B.Afwd; a.X = nullptr; a.Y = L.plan[0]; a.ncols = cplx_cols;
+        HIPCHK(launch_proj(NP, PROJ_C2P, a, L.stream));
+    }
+    {   // pass 2: G = (P^-1 T)^T ./ (lam_i + lam_j - xlambda)
+        Timed t(PK_P2P_DIV);
+        a.X = L.plan[0]; a.Y = L.plan[1]; a.ncols = planes * NP;
+        HIPCHK(launch_proj(NP, PROJ_P2P_DIV, a, L.stream));
+    }
+    if (bidx == 1) {  // Neumann gauge planes: |xlambda| <= 1e-10 (solvediag.f90:151)
+        GaugeArgs ga{};
+        ga.G = L.plan[1]; ga.prow = B.prow; ga.pdiag = B.pdiag; ga.wanted = B.wanted;
+        ga.gauge = B.gauge; ga.NP = NP; ga.g = g;
+        GaugeList gl{};
+        for (int f = 0; f < g.nf; ++f)
+            for (int k = 0; k < g.nmodes; ++k) {
+                const double xl = xlambda_of(g.alambda[f], g.alp, g.mode0 + k);
+                if (!(std::fabs(xl) <= 1.0e-10)) continue;
+                const int fm = f * g.nmodes + k;
+                for (int part = 0; part < 2; ++part) {
+                    if (part == 1 && g.mode0 + k == 0) continue;  // Im skipped at k=0
+                    gl.plane[gl.count++] = 2 * fm + part;
+                    if (gl.count == kMaxGaugePlanes) {
+                        Timed t(PK_GAUGE);
+                        HIPCHK(launch_gauge(ga, gl, L.stream));
+                        gl.count = 0;
+                    }
+                }
+            }
+        if (gl.count) {
+            Timed t(PK_GAUGE);
+            HIPCHK(launch_gauge(ga, gl, L.stream));
+        }
+    }
+    {   // pass 3: T = (P G)^T
+        Timed t(PK_P2P);
+        a.Afrag = B.Abwd; a.X = L.plan[1]; a.Y = L.plan[0]; a.ncols = planes * NP;
+        HIPCHK(launch_proj(NP, PROJ_P2P, a, L.stream));
+    }
+    {   // pass 4: U = (P T)^T = P G P^T into the field interior
+        Timed t(PK_P2C);
+        a.X = L.plan[0]; a.Y = nullptr; a.ncols = cplx_cols;
+        HIPCHK(launch_proj(NP, PROJ_P2C, a, L.stream));
+    }
+    {
+        Timed t(PK_BOUNDARY);
+        BoundaryArgs ba{};
+        ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
+        ba.mode = bidx == 1 ? BND_NEUMANN : bmode;
+        HIPCHK(launch_boundary(ba, L.stream));
+    }
+    return OK;
+}
+
+int check_common(int m, int nmodes)
+{
+    if (m < kMinM || m > kMaxM) return fail(ERR_UNSUPPORTED, "m=%d outside [%d, %d]", m, kMinM, kMaxM);
+    if (nmodes < 1) return fail(ERR_ARG, "nmodes=%d < 1", nmodes);
+    return OK;
+}
+
+// Solve a batch of device fields: group by boundary type, one run per group.
+int solve_fields(int nf, const duct_devptr_t* rhs, const duct_devptr_t* sol, const char* bounds,
+                 const double* alambda, int m, int nmodes, int ib, double alp, int dir_bmode)
+{
+    if (nf < 1 || nf > kMaxFields) return fail(ERR_ARG, "nfields=%d outside [1, %d]", nf, kMaxFields);
+    int st = check_common(m, nmodes);
+    if (st) return st;
+    for (int bidx = 0; bidx < 2; ++bidx) {
+        FieldGroup g{};
+        g.nmodes = nmodes; g.mode0 = ib; g.m = m; g.n = m - 2; g.alp = alp;
+        for (int f = 0; f < nf; ++f) {
+            const int bi = bound_index(bounds[f]);
+            if (bi < 0) return fail(ERR_ARG, "bound '%c' of field %d is neither 'd' nor 'n'", bounds[f], f);
+            if (bi != bidx) continue;
+            if (!rhs[f] || !sol[f]) return fail(ERR_ARG, "null device pointer for field %d", f);
+            g.in[g.nf] = (const d2v*)rhs[f];
+            g.out[g.nf] = (d2v*)sol[f];
+            g.alambda[g.nf] = alambda[f];
+            ++g.nf;
+        }
+        if (g.nf == 0) continue;
+        st = run_group(g, bidx, dir_bmode);
+        if (st) return st;
+    }
+    return OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int duct_init_(const int* device)
+{
+    if (!device) return fail(ERR_ARG, "null device");
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (*device < 0 || *device >= count) return fail(ERR_ARG, "device %d not in [0, %d)", *device, count);
+    HIPCHK(hipSetDevice(*device));
+    L.device = *device;
+    L.init = true;
+    return OK;
+}
+
+int duct_shutdown_(void)
+{
+    if (!L.init) return OK;
+    (void)hipStreamSynchronize(L.stream);
+    for (auto& B : L.basis) {
+        for (double** p : {&B.Afwd, &B.Abwd, &B.lam, &B.DC, &B.AM1, &B.D2b, &B.prow})
+            if (*p) { (void)hipFree(*p); *p = nullptr; }
+        B = DevBasis{};
+    }
+    for (auto& p : L.plan) if (p) { (void)hipFree(p); p = nullptr; }
+    for (auto& p : L.hfield) if (p) { (void)hipFree(p); p = nullptr; }
+    L.plan_bytes = L.hfield_bytes = 0;
+    for (auto& r : L.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto& e : L.pool) (void)hipEventDestroy(e);
+    L.recs.clear();
+    L.pool.clear();
+    L.prof = false;
+    L.stream = nullptr;
+    L.init = false;
+    return OK;
+}
+
+int duct_set_stream_(const duct_devptr_t* stream)
+{
+    if (!stream) return fail(ERR_ARG, "null stream pointer");
+    int st = ensure_init();
+    if (st) return st;
+    L.stream = (hipStream_t)(*stream);
+    return OK;
+}
+
+int duct_synchronize_(void)
+{
+    int st = ensure_init();
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(L.stream));
+    return OK;
+}
+
+int duct_last_error_(char* buf, const int* len)
+{
+    if (!buf || !len || *len <= 0) return ERR_ARG;
+    snprintf(buf, (size_t)*len, "%s", L.err.c_str());
+    return OK;
+}
+
+int duct_setup_(const char* bound, const int* m_, const double* P, const double* Pinv, const double* lam,
+                const double* DC, const double* AM1, const int* iofzero, const double* wanted,
+                size_t bound_len)
+{
+    if (!bound || !m_ || !P || !Pinv || !lam || !DC) return fail(ERR_ARG, "null argument to duct_setup_");
+    (void)bound_len;
+    const int bidx = bound_index(bound[0]);
+    if (bidx < 0) return fail(ERR_ARG, "in pseudomat error: bound '%c'", bound[0]);
+    const int m = *m_;
+    if (m < kMinM || m > kMaxM) return fail(ERR_UNSUPPORTED, "m=%d outside [%d, %d]", m, kMinM, kMaxM);
+    if (bidx == 1 && (!AM1 || !iofzero || !wanted)) return fail(ERR_ARG, "Neumann setup needs AM1, iofzero, wanted");
+    const int n = m - 2, NP = padded_n(n);
+    if (bidx == 1 && (*iofzero < 1 || *iofzero > n)) return fail(ERR_ARG, "iofzero=%d outside [1, %d]", *iofzero, n);
+    int st = ensure_init();
+    if (st) return st;
+
+    DevBasis& B = L.basis[bidx];
+    HIPCHK(hipStreamSynchronize(L.stream));
+    for (double** p : {&B.Afwd, &B.Abwd, &B.lam, &B.DC, &B.AM1, &B.D2b, &B.prow})
+        if (*p) { HIPCHK(hipFree(*p)); *p = nullptr; }
+    B = DevBasis{};
+
+    std::vector<double> fwd((size_t)NP * NP), bwd((size_t)NP * NP), lp(NP, 0.0), d2b((size_t)m * 2, 0.0),
+        prow(NP, 0.0), am1(16, 0.0);
+    build_frag(Pinv, n, n, NP, fwd.data());
+    build_frag(P, n, n, NP, bwd.data());
+    for (int i = 0; i < n; ++i) lp[i] = lam[i];
+    for (int i = 0; i < m; ++i) {  // D2(:,1) and D2(:,m) of D2 = DC*DC (pseudomat.f90:110, :225-230)
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = 0; k < m; ++k) {
+            s0 += DC[i + (size_t)k * m] * DC[k];
+            s1 += DC[i + (size_t)k * m] * DC[k + (size_t)(m - 1) * m];
+        }
+        d2b[i] = s0;
+        d2b[i + m] = s1;
+    }
+    if (bidx == 1) {
+        const int io = *iofzero - 1;
+        for (int k = 0; k < n; ++k) prow[k] = P[io + (size_t)k * n];
+        B.pdiag = P[io + (size_t)io * n];
+        B.wanted = *wanted;
+        B.gauge = io;
+        for (int e = 0; e < 16; ++e) am1[e] = AM1[e];
+    }
+    auto up = [&](double*& dst, const double* src, size_t count) -> int {
+        HIPCHK(hipMalloc((void**)&dst, count * sizeof(double)));
+        HIPCHK(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice));
+        return OK;
+    };
+    if ((st = up(B.Afwd, fwd.data(), fwd.size()))) return st;
+    if ((st = up(B.Abwd, bwd.data(), bwd.size()))) return st;
+    if ((st = up(B.lam, lp.data(), lp.size()))) return st;
+    if ((st = up(B.DC, DC, (size_t)m * m))) return st;
+    if ((st = up(B.AM1, am1.data(), 16))) return st;
+    if ((st = up(B.D2b, d2b.data(), d2b.size()))) return st;
+    if ((st = up(B.prow, prow.data(), prow.size()))) return st;
+    B.m = m;
+    B.n = n;
+    B.NP = NP;
+    B.ready = true;
+    return OK;
+}
+
+int duct_helm3d_dev_(const duct_devptr_t* rhs, const duct_devptr_t* sol, const int* m, const int* nmodes,
+                     const int* ib, const double* alp, const double* alambda, const char* bound,
+                     size_t bound_len)
+{
+    if (!rhs || !sol || !m || !nmodes || !ib || !alp || !alambda || !bound)
+        return fail(ERR_ARG, "null argument to duct_helm3d_dev_");
+    (void)bound_len;
+    int st = ensure_init();
+    if (st) return st;
+    return solve_fields(1, rhs, sol, bound, alambda, *m, *nmodes, *ib, *alp, BND_DIR_ZERO);
+}
+
+int duct_helm3d_batch_dev_(const int* nfields, const duct_devptr_t* rhs, const duct_devptr_t* sol,
+                           const char* bounds, const double* alambda, const int* m, const int* nmodes,
+                           const int* ib, const double* alp, size_t bounds_len)
+{
+    if (!nfields || !rhs || !sol || !bounds || !alambda || !m || !nmodes || !ib || !alp)
+        return fail(ERR_ARG, "null argument to duct_helm3d_batch_dev_");
+    if (bounds_len && (size_t)*nfields > bounds_len)
+        return fail(ERR_ARG, "bounds has %zu characters for %d fields", bounds_len, *nfields);
+    int st = ensure_init();
+    if (st) return st;
+    return solve_fields(*nfields, rhs, sol, bounds, alambda, *m, *nmodes, *ib, *alp, BND_DIR_ZERO);
+}
+
+int duct_helm3d_(double* rhs, double* sol, const int* m, const int* nmodes, const int* ib, const double* alp,
+                 const double* alambda, const char* bound, size_t bound_len)
+{
+    if (!rhs || !sol || !m || !nmodes || !ib || !alp || !alambda || !bound)
+        return fail(ERR_ARG, "null argument to duct_helm3d_");
+    (void)bound_len;
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    const size_t bytes = (size_t)(*nmodes) * (*m) * (*m) * 2 * sizeof(double);
+    if (bytes > L.hfield_bytes || !L.hfield[0]) {
+        size_t c0 = L.hfield_bytes, c1 = L.hfield_bytes;
+        if ((st = ensure_buf(L.hfield[0], c0, bytes))) return st;
+        if ((st = ensure_buf(L.hfield[1], c1, bytes))) return st;
+        L.hfield_bytes = bytes;
+    }
+    HIPCHK(hipMemcpyAsync(L.hfield[0], rhs, bytes, hipMemcpyHostToDevice, L.stream));
+    duct_devptr_t r = (duct_devptr_t)L.hfield[0], s = (duct_devptr_t)L.hfield[1];
+    if ((st = solve_fields(1, &r, &s, bound, alambda, *m, *nmodes, *ib, *alp, BND_DIR_ZERO))) return st;
+    HIPCHK(hipMemcpyAsync(sol, L.hfield[1], bytes, hipMemcpyDeviceToHost, L.stream));
+    if (bound[0] == 'd') HIPCHK(hipMemcpyAsync(rhs, L.hfield[0], bytes, hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(hipStreamSynchronize(L.stream));
+    return OK;
+}
+
+// Shared by duct_int2d3d_ and duct_solvediag_: one complex plane held in
+// hfield[0] (in) and solved into hfield[1] (out).
+static int solve_plane(const char* bound, double xlambda, int imode)
+{
+    const int bidx = bound_index(bound[0]);
+    if (bidx < 0) return fail(ERR_ARG, "in solvediag error in BC: bound '%c'", bound[0]);
+    DevBasis& B = L.basis[bidx];
+    if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bound[0]);
+    FieldGroup g{};
+    g.nf = 1; g.nmodes = 1; g.mode0 = imode; g.m = B.m; g.n = B.n; g.alp = 0.0;
+    g.in[0] = L.hfield[0];
+    g.out[0] = L.hfield[1];
+    g.alambda[0] = xlambda;
+    if (bidx == 0) {
+        Timed t(PK_BCCORR);
+        BcCorrectArgs bc{};
+        bc.g = g;
+        bc.D2b = B.D2b;
+        HIPCHK(launch_bc_correct(bc, L.stream));
+    }
+    return run_group(g, bidx, BND_DIR_KEEP);
+}
+
+static int plane_buffers(int m)
+{
+    const size_t bytes = (size_t)m * m * 2 * sizeof(double);
+    if (bytes > L.hfield_bytes || !L.hfield[0]) {
+        size_t c0 = L.hfield_bytes, c1 = L.hfield_bytes;
+        int st;
+        if ((st = ensure_buf(L.hfield[0], c0, bytes))) return st;
+        if ((st = ensure_buf(L.hfield[1], c1, bytes))) return st;
+        L.hfield_bytes = bytes;
+    }
+    return OK;
+}
+
+int duct_int2d3d_(double* sol, const double* rhs, const char* bound, const double* xlambda, const int* imode,
+                  size_t bound_len)
+{
+    if (!sol || !rhs || !bound || !xlambda || !imode) return fail(ERR_ARG, "null argument to duct_int2d3d_");
+    (void)bound_len;
+    int st = ensure_init();
+    if (st) return st;
+    const int bidx = bound_index(bound[0]);
+    if (bidx < 0) return fail(ERR_ARG, "mistake in bound '%c'", bound[0]);
+    const int m = L.basis[bidx].m;
+    if (!L.basis[bidx].ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bound[0]);
+    if ((st = plane_buffers(m))) return st;
+    const size_t bytes = (size_t)m * m * 2 * sizeof(double);
+    HIPCHK(hipMemcpyAsync(L.hfield[0], rhs, bytes, hipMemcpyHostToDevice, L.stream));
+    if ((st = solve_plane(bound, *xlambda, *imode))) return st;
+    HIPCHK(hipMemcpyAsync(sol, L.hfield[1], bytes, hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(hipStreamSynchronize(L.stream));
+    return OK;
+}
+
+int duct_solvediag_(double* U, const char* bound, const double* xlambda, size_t bound_len)
+{
+    if (!U || !bound || !xlambda) return fail(ERR_ARG, "null argument to duct_solvediag_");
+    (void)bound_len;
+    int st = ensure_init();
+    if (st) return st;
+    const int bidx = bound_index(bound[0]);
+    if (bidx < 0) return fail(ERR_ARG, "in solvediag error in BC: bound '%c'", bound[0]);
+    if (!L.basis[bidx].ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bound[0]);
+    const int m = L.basis[bidx].m;
+    if ((st = plane_buffers(m))) return st;
+    std::vector<double> c((size_t)m * m * 2, 0.0);
+    for (size_t e = 0; e < (size_t)m * m; ++e) c[2 * e] = U[e];
+    const size_t bytes = c.size() * sizeof(double);
+    HIPCHK(hipMemcpyAsync(L.hfield[0], c.data(), bytes, hipMemcpyHostToDevice, L.stream));
+    // a nonzero Fourier index keeps the (all-zero) imaginary plane ordinary
+    if ((st = solve_plane(bound, *xlambda, 1))) return st;
+    HIPCHK(hipMemcpyAsync(c.data(), L.hfield[1], bytes, hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(hipStreamSynchronize(L.stream));
+    for (size_t e = 0; e < (size_t)m * m; ++e) U[e] = c[2 * e];
+    return OK;
+}
+
+int duct_prof_enable_(const int* on)
+{
+    if (!on) return fail(ERR_ARG, "null argument");
+    int st = ensure_init();
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(L.stream));
+    for (auto& r : L.recs) {
+        L.pool.push_back(r.a);
+        L.pool.push_back(r.b);
+    }
+    L.recs.clear();
+    L.prof = *on != 0;
+    return OK;
+}
+
+int duct_prof_read_(const int* nclass, double* ms, int* count)
+{
+    if (!nclass || !ms || !count) return fail(ERR_ARG, "null argument");
+    int st = ensure_init();
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(L.stream));
+    for (int k = 0; k < *nclass; ++k) { ms[k] = 0.0; count[k] = 0; }
+    for (auto& r : L.recs) {
+        if (r.kind >= *nclass) continue;
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+        ms[r.kind] += t;
+        count[r.kind] += 1;
+    }
+    return OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+/*
+ * duct_hip.h - C-ABI of libduct_hip.so, the MI355X (gfx950) implementation of
+ * duct's per-Fourier-mode cross-plane Helmholtz/Poisson solve stage.
+ *
+ * Conventions (the style of the reference's Fortran bindings, src/fortran.c
+ * and src/mytranspose_cuda.cu of ando2800/duct):
+ *   - lowercase names with a trailing underscore, every argument by pointer,
+ *     so gfortran/flang can call them through implicit interfaces;
+ *   - a Fortran CHARACTER argument adds a trailing hidden length (size_t, by
+ *     value) at the end of the list;
+ *   - device addresses travel as duct_devptr_t (an integer holding the
+ *     address, like devptr_t in src/fortran.h:55);
+ *   - every entry point returns an int status: 0 = success, otherwise one of
+ *     DUCT_ERR_*; duct_last_error_ returns the message.  (The reference's GPU
+ *     entry points always return 0, mytranspose_cuda.cu:149,178,195.)
+ *   - complex fields are complex(8) (m, m, nmodes), y fastest, i.e. the
+ *     reference's rhs(my,mz,ib:ie) (helm3d.f90:29).
+ * One host thread drives the library; calls are ordered on the stream set by
+ * duct_set_stream_ (default: the null stream).
+ */
+#ifndef DUCT_HIP_H
+#define DUCT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef size_t duct_devptr_t;
+
+#define DUCT_OK 0
+#define DUCT_ERR_ARG 1
+#define DUCT_ERR_NOT_SETUP 2
+#define DUCT_ERR_HIP 3
+#define DUCT_ERR_ALLOC 4
+#define DUCT_ERR_UNSUPPORTED 5
+
+/* ---- runtime ------------------------------------------------------------ */
+
+/* Select the device and create the library state.
+ * Replaces CUBLAS_INIT (src/fortran.c:71) as called at src/main1.f90:149-150. */
+int duct_init_(const int *device);
+
+/* Free every device buffer the library owns.  Replaces CUBLAS_SHUTDOWN. */
+int duct_shutdown_(void);
+
+/* Stream for all later launches (a hipStream_t held as an integer, 0 = null
+ * stream).  Replaces cuda_stream_create_/cublas_set_stream_
+ * (src/mytranspose_cuda.cu:198-207). */
+int duct_set_stream_(const duct_devptr_t *stream);
+
+/* Wait for all work the library queued on its stream. */
+int duct_synchronize_(void);
+
+/* Copy the last error message (NUL-terminated, truncated to *len bytes). */
+int duct_last_error_(char *buf, const int *len);
+
+/* ---- one-time setup ----------------------------------------------------- */
+
+/* Upload the eigenbasis of one boundary type after the host eigensolve.
+ * Replaces the GPU-upload tails of pseudomat (src/pseudomat.f90:188-221 for
+ * 'n', :265-298 for 'd'); the caller keeps its own LAPACK dgeev/dgetri.
+ *   bound    'd' (Dirichlet) or 'n' (Neumann)
+ *   m        CGL points per direction (my == mz), 4 <= m <= 514
+ *   P        n x n right eigenvectors, n = m-2 (PD or PN)
+ *   Pinv     n x n inverse (PDM1 or PNM1)
+ *   lam      n eigenvalues (ED or EN)
+ *   DC       m x m collocation derivative matrix (pseudomat.f90:90-107)
+ *   AM1      4 x 4 inverse corner matrix (Neumann only, may be NULL for 'd')
+ *   iofzero  1-based eigen-index of the Neumann null mode (ignored for 'd')
+ *   wanted   gauge value at (iofzero+1, iofzero+1) (ignored for 'd')       */
+int duct_setup_(const char *bound, const int *m, const double *P, const double *Pinv,
+                const double *lam, const double *DC, const double *AM1, const int *iofzero,
+                const double *wanted, size_t bound_len);
+
+/* ---- the solve stage ---------------------------------------------------- */
+
+/* helm3d on device-resident fields (src/helm3d.f90:2, ipro >= 1):
+ * solve (D_y^2 + D_z^2 - xlambda) sol = rhs for modes ib .. ib+nmodes-1 with
+ * xlambda = alambda + (alp*imode)^2 (helm3d.f90:67, fourier.F:201-203).
+ * For bound 'd' the boundary of rhs is zeroed in place (helm3d.f90:48-60) and
+ * sol's boundary is zero; for 'n' sol's boundary comes from the Neumann edge
+ * and corner equations.  rhs and sol may be the same field.               */
+int duct_helm3d_dev_(const duct_devptr_t *rhs, const duct_devptr_t *sol, const int *m,
+                     const int *nmodes, const int *ib, const double *alp, const double *alambda,
+                     const char *bound, size_t bound_len);
+
+/* The same over nfields (<= 8) fields in one pass sequence: rhs[f], sol[f]
+ * device fields, bounds[f] in {'d','n'} (a CHARACTER*(nfields)), alambda[f].
+ * Fields sharing a boundary type share every launch.  This is the u, v, w
+ * (Dirichlet) + p (Neumann) batch of src/main1.f90:617-643.                */
+int duct_helm3d_batch_dev_(const int *nfields, const duct_devptr_t *rhs, const duct_devptr_t *sol,
+                           const char *bounds, const double *alambda, const int *m,
+                           const int *nmodes, const int *ib, const double *alp,
+                           size_t bounds_len);
+
+/* helm3d on host arrays (drop-in for the Fortran subroutine body): uploads
+ * rhs, solves on the device, downloads sol and the boundary-zeroed rhs.   */
+int duct_helm3d_(double *rhs, double *sol, const int *m, const int *nmodes, const int *ib,
+                 const double *alp, const double *alambda, const char *bound, size_t bound_len);
+
+/* int2d3d (src/int2d3d.f90:2) on host arrays: one complex plane, real part
+ * solved always, imaginary part skipped when imode == 0 (sol imag = 0).    */
+int duct_int2d3d_(double *sol, const double *rhs, const char *bound, const double *xlambda,
+                  const int *imode, size_t bound_len);
+
+/* solvediag (src/solvediag.f90:2) on one host real plane U(m,m), in place:
+ * interior input = right-hand side, boundary input = Dirichlet data ('d',
+ * with the boundary correction of :37-45) or ignored ('n'); output = the
+ * solution including the Neumann boundary.  Also the entry for direct
+ * callers such as precomp_mass_flow (src/operators.F:2276).               */
+int duct_solvediag_(double *U, const char *bound, const double *xlambda, size_t bound_len);
+
+/* ---- kernel timing (hipEvents on the library stream) -------------------- */
+
+/* Enable (1) / disable (0) per-launch event timing and clear the records. */
+int duct_prof_enable_(const int *on);
+/* Synchronise and return, for each kernel class k < *nclass, the summed
+ * device time in ms and the launch count.  Classes: 0 proj_c2p,
+ * 1 proj_p2p_div, 2 gauge, 3 proj_p2p, 4 proj_p2c, 5 boundary, 6 bc_correct. */
+int duct_prof_read_(const int *nclass, double *ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DUCT_HIP_H */
