@@ -25,25 +25,31 @@ __device__ __forceinline__ d2v wave_sum2(d2v v)
 // ---------------------------------------------------------------------------
 // Gauge.  For a plane G = P^-1 F P^-T ./ den with G(a,a) = 0, set
 //   G(a,a) = (wanted - sum_l P(a,l) sum_k P(a,k) G(l,k)) / P(a,a)^2
-// so that (P G P^T)(a,a) = wanted.  One workgroup per listed plane.
-__global__ __launch_bounds__(256) void k_gauge(GaugeArgs a, GaugeList list)
+// so that (P G P^T)(a,a) = wanted.  One 1024-thread workgroup per listed
+// plane: lanes run down the contiguous l index, the 16 waves split k.
+constexpr int kAuxThreads = 1024;
+constexpr int kAuxWaves = kAuxThreads / 64;
+
+__global__ __launch_bounds__(kAuxThreads) void k_gauge(GaugeArgs a, GaugeList list)
 {
     const int plane = list.plane[blockIdx.x];
     const int NP = a.NP, n = a.g.n, io = a.gauge;
     double* G = a.G + (size_t)plane * NP * NP;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double part = 0.0;
-    for (int l = tid; l < n; l += 256) {
-        double val = 0.0;
-        for (int k = 0; k < n; ++k) val += a.prow[k] * G[l + (size_t)k * NP];
-        part += a.prow[l] * val;
+    for (int k = wave; k < n; k += kAuxWaves) {
+        const double pk = a.prow[k];
+        const double* col = G + (size_t)k * NP;
+#pragma unroll 4
+        for (int l = lane; l < n; l += 64) part += a.prow[l] * col[l] * pk;
     }
-    __shared__ double red[4];
+    __shared__ double red[kAuxWaves];
     part = wave_sum(part);
-    if ((tid & 63) == 0) red[tid >> 6] = part;
+    if (lane == 0) red[wave] = part;
     __syncthreads();
     if (tid == 0) {
-        const double valore = (red[0] + red[1]) + (red[2] + red[3]);
+        double valore = 0.0;
+        for (int w = 0; w < kAuxWaves; ++w) valore += red[w];
         G[io + (size_t)io * NP] = (a.wanted - valore) / (a.pdiag * a.pdiag);
     }
 }
@@ -51,13 +57,22 @@ __global__ __launch_bounds__(256) void k_gauge(GaugeArgs a, GaugeList list)
 hipError_t launch_gauge(const GaugeArgs& a, const GaugeList& list, hipStream_t st)
 {
     if (list.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gauge, dim3(list.count), dim3(256), 0, st, a, list);
+    hipLaunchKernelGGL(k_gauge, dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Boundary of each complex plane after pass 4.  One workgroup per (field, mode).
-__global__ __launch_bounds__(256) void k_boundary(BoundaryArgs a)
+// Boundary of each complex plane after pass 4.  One 1024-thread workgroup per
+// (field, mode).
+//
+// Neumann (solvediag.f90:251-322): with det = DC(1,1)DC(m,m) - DC(1,m)DC(m,1),
+//   U(i,1) = (-DC(m,m) sb + DC(1,m) st)/det, U(i,m) = (DC(m,1) sb - DC(1,1) st)/det,
+//   sb = sum_j DC(1,j) U(i,j), st = sum_j DC(m,j) U(i,j) over interior j,
+// and the same along y for U(1,j), U(m,j).  The four corner right-hand sides
+// ff(1..4) are linear in those edge values, so every thread that produces an
+// edge value adds its share to ff directly and the block reduces ff at the
+// end; the corners are AM1 * ff.
+__global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
 {
     const FieldGroup& g = a.g;
     const int fm = blockIdx.x;
@@ -66,17 +81,17 @@ __global__ __launch_bounds__(256) void k_boundary(BoundaryArgs a)
     const size_t mm = (size_t)m * m;
     d2v* out = g.out[f] + (size_t)mode * mm;
     d2v* in = const_cast<d2v*>(g.in[f]) + (size_t)mode * mm;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     if (a.mode == BND_DIR_ZERO || a.mode == BND_DIR_KEEP) {
         const bool zero = a.mode == BND_DIR_ZERO;
-        for (int e = tid; e < 4 * m; e += 256) {
+        for (int e = tid; e < 4 * m; e += kAuxThreads) {
             const int side = e / m, t = e % m;
             size_t idx;
-            if (side == 0) idx = (size_t)t;                         // z = 1 column
+            if (side == 0) idx = (size_t)t;                            // z = 1 column
             else if (side == 1) idx = (size_t)t + (size_t)(m - 1) * m; // z = m
-            else if (side == 2) idx = (size_t)t * m;                // y = 1 row
-            else idx = (size_t)(m - 1) + (size_t)t * m;             // y = m row
+            else if (side == 2) idx = (size_t)t * m;                   // y = 1 row
+            else idx = (size_t)(m - 1) + (size_t)t * m;                // y = m row
             d2v v = {0.0, 0.0};
             if (!zero) v = in[idx];
             if (zero) in[idx] = v;
@@ -86,31 +101,47 @@ __global__ __launch_bounds__(256) void k_boundary(BoundaryArgs a)
         return;
     }
 
-    // Neumann: U(i,1), U(i,m) from the two DC rows along z, U(1,j), U(m,j) along y.
-    __shared__ d2v eb[kMaxM], et[kMaxM], el[kMaxM], er[kMaxM];
     const double* DC = a.DC;
     const double DC00 = DC[0], DCmm = DC[(m - 1) + (size_t)(m - 1) * m];
     const double DC0m = DC[(size_t)(m - 1) * m], DCm0 = DC[m - 1];
     const double det = DC00 * DCmm - DC0m * DCm0;
+    d2v f1 = {0, 0}, f2 = {0, 0}, f3 = {0, 0}, f4 = {0, 0};
 
-    // bottom/top lines (z = 1, z = m): sums along z, one thread per y
-    for (int i = 1 + tid; i < m - 1; i += 256) {
+    // z = 1 and z = m lines: sums along z for every interior y.  Four threads
+    // per y split the z range; lanes of a wave take consecutive y (coalesced).
+    __shared__ d2v part[4][256][2];
+    const int ty = tid & 255, zc = tid >> 8;
+    for (int yb = 0; yb < m - 2; yb += 256) {
+        const int i = 1 + yb + ty;
         d2v sb = {0.0, 0.0}, stp = {0.0, 0.0};
-        for (int j = 1; j < m - 1; ++j) {
-            const d2v u = out[(size_t)i + (size_t)j * m];
-            sb += DC[(size_t)j * m] * u;
-            stp += DC[(m - 1) + (size_t)j * m] * u;
+        if (i < m - 1) {
+#pragma unroll 4
+            for (int j = 1 + zc; j < m - 1; j += 4) {
+                const d2v u = out[(size_t)i + (size_t)j * m];
+                sb += DC[(size_t)j * m] * u;
+                stp += DC[(m - 1) + (size_t)j * m] * u;
+            }
         }
-        const d2v vb = (-DCmm * sb + DC0m * stp) / det;
-        const d2v vt = (DCm0 * sb - DC00 * stp) / det;
-        eb[i] = vb;
-        et[i] = vt;
-        out[(size_t)i] = vb;
-        out[(size_t)i + (size_t)(m - 1) * m] = vt;
+        part[zc][ty][0] = sb;
+        part[zc][ty][1] = stp;
+        __syncthreads();
+        if (zc == 0 && i < m - 1) {
+            sb = (part[0][ty][0] + part[1][ty][0]) + (part[2][ty][0] + part[3][ty][0]);
+            stp = (part[0][ty][1] + part[1][ty][1]) + (part[2][ty][1] + part[3][ty][1]);
+            const d2v vb = (-DCmm * sb + DC0m * stp) / det;
+            const d2v vt = (DCm0 * sb - DC00 * stp) / det;
+            out[(size_t)i] = vb;
+            out[(size_t)i + (size_t)(m - 1) * m] = vt;
+            const double d0 = DC[(size_t)i * m], dm = DC[(m - 1) + (size_t)i * m];
+            f1 -= d0 * vb;
+            f2 += dm * vb;
+            f3 -= dm * vt;
+            f4 -= d0 * vt;
+        }
+        __syncthreads();
     }
-    // left/right lines (y = 1, y = m): sums along y, one wave per z column
-    const int lane = tid & 63, wave = tid >> 6;
-    for (int j = 1 + wave; j < m - 1; j += 4) {
+    // y = 1 and y = m lines: sums along y (contiguous), one wave per z
+    for (int j = 1 + wave; j < m - 1; j += kAuxWaves) {
         d2v sl = {0.0, 0.0}, sr = {0.0, 0.0};
         for (int i = 1 + lane; i < m - 1; i += 64) {
             const d2v u = out[(size_t)i + (size_t)j * m];
@@ -122,37 +153,40 @@ __global__ __launch_bounds__(256) void k_boundary(BoundaryArgs a)
         if (lane == 0) {
             const d2v vl = (-DCmm * sl + DC0m * sr) / det;
             const d2v vr = (DCm0 * sl - DC00 * sr) / det;
-            el[j] = vl;
-            er[j] = vr;
             out[(size_t)j * m] = vl;
             out[(m - 1) + (size_t)j * m] = vr;
+            const double d0 = DC[(size_t)j * m], dm = DC[(m - 1) + (size_t)j * m];
+            f1 -= d0 * vl;
+            f2 -= d0 * vr;
+            f3 -= dm * vr;
+            f4 += dm * vl;
         }
     }
+    // corners: reduce ff over the block, U_corner = AM1 * ff
+    __shared__ d2v fred[kAuxWaves][4];
+    f1 = wave_sum2(f1);
+    f2 = wave_sum2(f2);
+    f3 = wave_sum2(f3);
+    f4 = wave_sum2(f4);
+    if (lane == 0) {
+        fred[wave][0] = f1;
+        fred[wave][1] = f2;
+        fred[wave][2] = f3;
+        fred[wave][3] = f4;
+    }
     __syncthreads();
-    // corners from the four summed Neumann equations: wave 0
-    if (wave == 0) {
-        d2v f1 = {0, 0}, f2 = {0, 0}, f3 = {0, 0}, f4 = {0, 0};
-        for (int i = 1 + lane; i < m - 1; i += 64) {
-            const double d0 = DC[(size_t)i * m], dm = DC[(m - 1) + (size_t)i * m];
-            f1 -= d0 * (eb[i] + el[i]);
-            f2 += -d0 * er[i] + dm * eb[i];
-            f3 -= dm * (et[i] + er[i]);
-            f4 += -d0 * et[i] + dm * el[i];
-        }
-        f1 = wave_sum2(f1);
-        f2 = wave_sum2(f2);
-        f3 = wave_sum2(f3);
-        f4 = wave_sum2(f4);
-        if (lane < 4) {
-            const double* A = a.AM1;  // 4x4 column-major, row `lane`
-            const d2v v = A[lane] * f1 + A[lane + 4] * f2 + A[lane + 8] * f3 + A[lane + 12] * f4;
-            size_t idx;
-            if (lane == 0) idx = 0;                                        // U(1,1)
-            else if (lane == 1) idx = (size_t)(m - 1);                     // U(m,1)
-            else if (lane == 2) idx = (size_t)(m - 1) + (size_t)(m - 1) * m; // U(m,m)
-            else idx = (size_t)(m - 1) * m;                                // U(1,m)
-            out[idx] = v;
-        }
+    if (tid < 4) {
+        d2v ff[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        for (int w = 0; w < kAuxWaves; ++w)
+            for (int c = 0; c < 4; ++c) ff[c] += fred[w][c];
+        const double* A = a.AM1;  // 4x4 column-major, row `tid`
+        const d2v v = A[tid] * ff[0] + A[tid + 4] * ff[1] + A[tid + 8] * ff[2] + A[tid + 12] * ff[3];
+        size_t idx;
+        if (tid == 0) idx = 0;                                          // U(1,1)
+        else if (tid == 1) idx = (size_t)(m - 1);                       // U(m,1)
+        else if (tid == 2) idx = (size_t)(m - 1) + (size_t)(m - 1) * m; // U(m,m)
+        else idx = (size_t)(m - 1) * m;                                 // U(1,m)
+        out[idx] = v;
     }
 }
 
@@ -160,7 +194,7 @@ hipError_t launch_boundary(const BoundaryArgs& a, hipStream_t st)
 {
     const int planes = a.g.nf * a.g.nmodes;
     if (planes <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_boundary, dim3(planes), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_boundary, dim3(planes), dim3(kAuxThreads), 0, st, a);
     return hipGetLastError();
 }
 

@@ -55,7 +55,7 @@ __device__ __forceinline__ int vcol_re(int ql) {
 }
 
 template <int NP, int MODE>
-__global__ __launch_bounds__(256) void k_proj(ProjArgs a)
+__global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
 {
     using S = ProjShape<NP>;
     constexpr int NT = S::NT, TM = S::TM, LDB = S::LDB, NK = S::NK;
