@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/duct_hip.h"
+#include "aux.h"
 #include "common.h"
 #include "proj.h"
 
@@ -39,7 +40,8 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 
-enum ProfKind { PK_C2P = 0, PK_P2P_DIV = 1, PK_GAUGE = 2, PK_P2P = 3, PK_P2C = 4, PK_BOUNDARY = 5, PK_BCCORR = 6 };
+enum ProfKind { PK_C2P = 0, PK_P2P_DIV = 1, PK_GAUGE = 2, PK_P2P = 3, PK_P2C = 4, PK_BOUNDARY = 5, PK_BCCORR = 6,
+                PK_TRANSPOSE = 7, PK_INVDIAG = 8 };
 
 struct Lib {
     bool init = false;
@@ -547,6 +549,58 @@ int duct_prof_read_(const int* nclass, double* ms, int* count)
         ms[r.kind] += t;
         count[r.kind] += 1;
     }
+    return OK;
+}
+
+int duct_transpose_dev_(const int* Nx, const int* Ny, const int* nbatch, const double* alpha, const duct_devptr_t* A,
+                        const duct_devptr_t* C)
+{
+    if (!Nx || !Ny || !nbatch || !alpha || !A || !C) return fail(ERR_ARG, "null argument to duct_transpose_dev_");
+    if (*Nx < 0 || *Ny < 0 || *nbatch < 0) return fail(ERR_ARG, "negative size");
+    if (!*A || !*C) return fail(ERR_ARG, "null device pointer");
+    if (*A == *C) return fail(ERR_ARG, "in-place transpose is not supported (A == C)");
+    int st = ensure_init();
+    if (st) return st;
+    Timed t(PK_TRANSPOSE);
+    HIPCHK(launch_transpose_f64((const double*)*A, (double*)*C, *Nx, *Ny, *nbatch, *alpha, L.stream));
+    return OK;
+}
+
+static int pencil(int nrow, int ncol, duct_devptr_t src, duct_devptr_t dst)
+{
+    if (nrow < 0 || ncol < 0) return fail(ERR_ARG, "negative size");
+    if (!src || !dst) return fail(ERR_ARG, "null device pointer");
+    if (src == dst) return fail(ERR_ARG, "in-place pencil transpose is not supported");
+    int st = ensure_init();
+    if (st) return st;
+    Timed t(PK_TRANSPOSE);
+    HIPCHK(launch_transpose_c128((const d2v*)src, (d2v*)dst, nrow, ncol, 1, 1.0, L.stream));
+    return OK;
+}
+
+int duct_cxyz2yzx_dev_(const int* nx, const int* my, const int* nz, const duct_devptr_t* phys, const duct_devptr_t* phys2)
+{
+    if (!nx || !my || !nz || !phys || !phys2) return fail(ERR_ARG, "null argument to duct_cxyz2yzx_dev_");
+    // phys(0:mx1, my, kb:ke) -> phys2(my, kb:ke, 0:mx1): the (nx) x (my*nz) complex transpose
+    return pencil(*nx, (*my) * (*nz), *phys, *phys2);
+}
+
+int duct_cyzx2xyz_dev_(const int* nx, const int* my, const int* nz, const duct_devptr_t* phys2, const duct_devptr_t* phys)
+{
+    if (!nx || !my || !nz || !phys || !phys2) return fail(ERR_ARG, "null argument to duct_cyzx2xyz_dev_");
+    return pencil((*my) * (*nz), *nx, *phys2, *phys);
+}
+
+int duct_mat_invdiag_dev_(const int* Nx, const int* Ny, const double* xlam, const duct_devptr_t* F,
+                          const duct_devptr_t* ED2, const duct_devptr_t* C)
+{
+    if (!Nx || !Ny || !xlam || !F || !ED2 || !C) return fail(ERR_ARG, "null argument to duct_mat_invdiag_dev_");
+    if (*Nx < 0 || *Ny < 0) return fail(ERR_ARG, "negative size");
+    if (!*F || !*ED2 || !*C) return fail(ERR_ARG, "null device pointer");
+    int st = ensure_init();
+    if (st) return st;
+    Timed t(PK_INVDIAG);
+    HIPCHK(launch_invdiag((const double*)*F, (const double*)*ED2, (double*)*C, (long)(*Nx) * (*Ny), *xlam, L.stream));
     return OK;
 }
 

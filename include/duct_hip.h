@@ -112,13 +112,37 @@ int duct_int2d3d_(double *sol, const double *rhs, const char *bound, const doubl
  * callers such as precomp_mass_flow (src/operators.F:2276).               */
 int duct_solvediag_(double *U, const char *bound, const double *xlambda, size_t bound_len);
 
+/* ---- bandwidth-bound helpers (reference GPU glue) ----------------------- */
+
+/* C = alpha * A^T for nbatch consecutive Nx x Ny FP64 matrices (column-major,
+ * C is Ny x Nx).  Replaces transpose_gpu_ (src/mytranspose_cuda.cu:132-162,
+ * kernel transposeCoalesced :69-88), which ignores alpha (callers pass 1.0)
+ * and silently drops sizes that are not multiples of 32; any size works here.
+ * A and C must not overlap.                                                 */
+int duct_transpose_dev_(const int *Nx, const int *Ny, const int *nbatch, const double *alpha,
+                        const duct_devptr_t *A, const duct_devptr_t *C);
+
+/* Fourier <-> physical pencil transposes of complex(8) data:
+ * phys(0:nx-1, my, nz) -> phys2(my, nz, 0:nx-1) and back.
+ * Replace cxyz2yzx / cyzx2xyz (src/parallel.F:476-501, :504-529).           */
+int duct_cxyz2yzx_dev_(const int *nx, const int *my, const int *nz, const duct_devptr_t *phys,
+                       const duct_devptr_t *phys2);
+int duct_cyzx2xyz_dev_(const int *nx, const int *my, const int *nz, const duct_devptr_t *phys2,
+                       const duct_devptr_t *phys);
+
+/* C = F ./ (ED2 - xlam) elementwise over Nx*Ny FP64 values.
+ * Replaces mat_invdiag_gpu_ (src/mytranspose_cuda.cu:181-196, kernel :45-53). */
+int duct_mat_invdiag_dev_(const int *Nx, const int *Ny, const double *xlam, const duct_devptr_t *F,
+                          const duct_devptr_t *ED2, const duct_devptr_t *C);
+
 /* ---- kernel timing (hipEvents on the library stream) -------------------- */
 
 /* Enable (1) / disable (0) per-launch event timing and clear the records. */
 int duct_prof_enable_(const int *on);
 /* Synchronise and return, for each kernel class k < *nclass, the summed
  * device time in ms and the launch count.  Classes: 0 proj_c2p,
- * 1 proj_p2p_div, 2 gauge, 3 proj_p2p, 4 proj_p2c, 5 boundary, 6 bc_correct. */
+ * 1 proj_p2p_div, 2 gauge, 3 proj_p2p, 4 proj_p2c, 5 boundary, 6 bc_correct,
+ * 7 transpose, 8 invdiag. */
 int duct_prof_read_(const int *nclass, double *ms, int *count);
 
 #ifdef __cplusplus
