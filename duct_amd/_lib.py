@@ -29,6 +29,8 @@ SIGNATURES = {
     "duct_helm3d_": (_i, [_vp, _vp, _pi, _pi, _pi, _pd, _pd, _pc, _sz]),
     "duct_int2d3d_": (_i, [_vp, _vp, _pc, _pd, _pi, _sz]),
     "duct_solvediag_": (_i, [_vp, _pc, _pd, _sz]),
+    "duct_helm3d_mixed_batch_dev_": (_i, [_pi, _psz, _psz, _pc, _pd, _pi, _pi, _pi, _pd, _pi, _pi, _pi,
+                                          _vp, _sz]),
     "duct_transpose_dev_": (_i, [_pi, _pi, _pi, _pd, _psz, _psz]),
     "duct_cxyz2yzx_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),
     "duct_cyzx2xyz_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),
@@ -41,7 +43,7 @@ STATUS = {0: "OK", 1: "DUCT_ERR_ARG", 2: "DUCT_ERR_NOT_SETUP", 3: "DUCT_ERR_HIP"
           4: "DUCT_ERR_ALLOC", 5: "DUCT_ERR_UNSUPPORTED"}
 
 PROF_CLASSES = ["proj_c2p", "proj_p2p_div", "gauge", "proj_p2p", "proj_p2c", "boundary", "bc_correct",
-                "transpose", "invdiag"]
+                "transpose", "invdiag", "lp_c2p", "lp_p2p_div", "lp_p2p", "lp_p2c", "resid"]
 
 
 class DuctError(RuntimeError):

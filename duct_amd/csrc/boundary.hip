@@ -34,14 +34,21 @@ __global__ __launch_bounds__(kAuxThreads) void k_gauge(GaugeArgs a, GaugeList li
 {
     const int plane = list.plane[blockIdx.x];
     const int NP = a.NP, n = a.g.n, io = a.gauge;
-    double* G = a.G + (size_t)plane * NP * NP;
+    const size_t base = (size_t)plane * NP * NP;
+    const double* Gd = (const double*)a.G + base;
+    const float* Gf = (const float*)a.G + base;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double part = 0.0;
     for (int k = wave; k < n; k += kAuxWaves) {
         const double pk = a.prow[k];
-        const double* col = G + (size_t)k * NP;
+        const size_t col = (size_t)k * NP;
+        if (a.f32) {
 #pragma unroll 4
-        for (int l = lane; l < n; l += 64) part += a.prow[l] * col[l] * pk;
+            for (int l = lane; l < n; l += 64) part += a.prow[l] * (double)Gf[col + l] * pk;
+        } else {
+#pragma unroll 4
+            for (int l = lane; l < n; l += 64) part += a.prow[l] * Gd[col + l] * pk;
+        }
     }
     __shared__ double red[kAuxWaves];
     part = wave_sum(part);
@@ -50,7 +57,16 @@ __global__ __launch_bounds__(kAuxThreads) void k_gauge(GaugeArgs a, GaugeList li
     if (tid == 0) {
         double valore = 0.0;
         for (int w = 0; w < kAuxWaves; ++w) valore += red[w];
-        G[io + (size_t)io * NP] = (a.wanted - valore) / (a.pdiag * a.pdiag);
+        double target = a.wanted;
+        if (a.relative) {
+            const int fm = plane >> 1, part = plane & 1;
+            const int f = fm / a.g.nmodes, mode = fm % a.g.nmodes, m = a.g.m;
+            const d2v u = a.g.out[f][(size_t)mode * m * m + (size_t)(io + 1) + (size_t)(io + 1) * m];
+            target -= u[part];
+        }
+        const double v = (target - valore) / (a.pdiag * a.pdiag);
+        if (a.f32) ((float*)a.G)[base + io + (size_t)io * NP] = (float)v;
+        else ((double*)a.G)[base + io + (size_t)io * NP] = v;
     }
 }
 
@@ -58,6 +74,56 @@ hipError_t launch_gauge(const GaugeArgs& a, const GaugeList& list, hipStream_t s
 {
     if (list.count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gauge, dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
+    return hipGetLastError();
+}
+
+// r -= s p p^T, s = q^T r q, on one part (re or im) of each listed plane.
+__global__ __launch_bounds__(kAuxThreads) void k_nullproj(NullProjArgs a, GaugeList list)
+{
+    const int plane = list.plane[blockIdx.x];
+    const int fm = plane >> 1, part = plane & 1;
+    const int f = fm / a.nmodes, mode = fm % a.nmodes, m = a.m, n = a.n;
+    double* R = (double*)(a.r[f] + (size_t)mode * m * m) + part;   // stride 2 doubles
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double s = 0.0;
+    for (int k = wave; k < n; k += kAuxWaves) {
+        const double qk = a.q[k];
+        for (int l = lane; l < n; l += 64)
+            s += a.q[l] * R[2 * ((size_t)(l + 1) + (size_t)(k + 1) * m)] * qk;
+    }
+    __shared__ double red[kAuxWaves];
+    __shared__ double tot;
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double v = 0.0;
+        for (int w = 0; w < kAuxWaves; ++w) v += red[w];
+        tot = v;
+    }
+    __syncthreads();
+    const double st = tot;
+    double rmx = 0.0;
+    for (int k = wave; k < n; k += kAuxWaves) {
+        const double pk = a.p[k];
+        for (int l = lane; l < n; l += 64) {
+            double* x = R + 2 * ((size_t)(l + 1) + (size_t)(k + 1) * m);
+            const double v = *x - st * a.p[l] * pk;
+            *x = v;
+            rmx = fmax(rmx, fabs(v));
+        }
+    }
+    if (a.stats) {
+        for (int off = 32; off > 0; off >>= 1) rmx = fmax(rmx, __shfl_xor(rmx, off, 64));
+        if (lane == 0)
+            atomicMax(a.stats + 2 * a.slot[f], (unsigned long long)__builtin_bit_cast(unsigned long long, rmx));
+    }
+}
+
+hipError_t launch_nullproj(const NullProjArgs& a, const GaugeList& list, hipStream_t st)
+{
+    if (list.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nullproj, dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
     return hipGetLastError();
 }
 

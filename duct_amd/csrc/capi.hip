@@ -16,6 +16,7 @@
 #include "aux.h"
 #include "common.h"
 #include "proj.h"
+#include "proj_lp.h"
 
 using namespace duct;
 
@@ -31,6 +32,13 @@ struct DevBasis {
     double* AM1 = nullptr;   // 4 x 4
     double* D2b = nullptr;   // m x 2: D2(:,1), D2(:,m)
     double* prow = nullptr;  // P(iofzero, :), NP
+    void* lpf_fwd = nullptr; // P^-1, FP32 fragment layout (mixed solve)
+    void* lpf_bwd = nullptr; // P
+    void* lph_fwd = nullptr; // P^-1, 3 bf16 terms
+    void* lph_bwd = nullptr; // P
+    double* opfrag = nullptr;// interior operator D2B, FP64 fragment layout (residual)
+    double* qrow = nullptr;  // P^-1(iofzero, :), Neumann null projection
+    double* pcol = nullptr;  // P(:, iofzero)
     double pdiag = 0.0, wanted = 0.0;
     int gauge = -1;          // 0-based eigen-index of the null mode (Neumann)
 };
@@ -41,7 +49,8 @@ struct ProfRec {
 };
 
 enum ProfKind { PK_C2P = 0, PK_P2P_DIV = 1, PK_GAUGE = 2, PK_P2P = 3, PK_P2C = 4, PK_BOUNDARY = 5, PK_BCCORR = 6,
-                PK_TRANSPOSE = 7, PK_INVDIAG = 8 };
+                PK_TRANSPOSE = 7, PK_INVDIAG = 8, PK_LP_C2P = 9, PK_LP_P2P_DIV = 10, PK_LP_P2P = 11,
+                PK_LP_P2C = 12, PK_RESID = 13 };
 
 struct Lib {
     bool init = false;
@@ -52,6 +61,10 @@ struct Lib {
     size_t plan_bytes = 0;
     d2v* hfield[2] = {nullptr, nullptr};  // device fields for host-array entries
     size_t hfield_bytes = 0;
+    d2v* rfield = nullptr;                // residual fields of the mixed solve
+    size_t rfield_bytes = 0;
+    unsigned long long* stats = nullptr;  // residual maxima of the mixed solve
+    size_t stats_bytes = 0;
     bool prof = false;
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
@@ -59,6 +72,21 @@ struct Lib {
 };
 
 Lib L;
+
+template <class P>
+void free_ptr(P*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_basis(DevBasis& B)
+{
+    free_ptr(B.Afwd); free_ptr(B.Abwd); free_ptr(B.lam); free_ptr(B.DC); free_ptr(B.AM1);
+    free_ptr(B.D2b); free_ptr(B.prow); free_ptr(B.lpf_fwd); free_ptr(B.lpf_bwd); free_ptr(B.lph_fwd);
+    free_ptr(B.lph_bwd); free_ptr(B.opfrag); free_ptr(B.qrow); free_ptr(B.pcol);
+    B = DevBasis{};
+}
 
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...)
@@ -189,7 +217,7 @@ int run_group(const FieldGroup& g, int bidx, int bmode)
     }
     if (bidx == 1) {  // Neumann gauge planes: |xlambda| <= 1e-10 (solvediag.f90:151)
         GaugeArgs ga{};
-        ga.G = L.plan[1]; ga.prow = B.prow; ga.pdiag = B.pdiag; ga.wanted = B.wanted;
+        ga.G = L.plan[1]; ga.f32 = 0; ga.prow = B.prow; ga.pdiag = B.pdiag; ga.wanted = B.wanted;
         ga.gauge = B.gauge; ga.NP = NP; ga.g = g;
         GaugeList gl{};
         for (int f = 0; f < g.nf; ++f)
@@ -228,6 +256,176 @@ int run_group(const FieldGroup& g, int bidx, int bmode)
         ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
         ba.mode = bidx == 1 ? BND_NEUMANN : bmode;
         HIPCHK(launch_boundary(ba, L.stream));
+    }
+    return OK;
+}
+
+int check_common(int m, int nmodes);
+
+// Low-precision pass chain for one field group: sol interior = (or +=) S(in).
+int run_group_lp(const FieldGroup& g, int bidx, int scheme, int accumulate, double wanted)
+{
+    DevBasis& B = L.basis[bidx];
+    const int NP = B.NP;
+    const long cplx_cols = (long)g.nf * g.nmodes * NP;
+    const long planes = (long)g.nf * g.nmodes * 2;
+    int st = ensure_planar((size_t)planes * NP * NP * sizeof(float));
+    if (st) return st;
+    const bool f32 = scheme == LP_F32;
+    ProjLpArgs a{};
+    a.g = g;
+    a.lam = B.lam;
+    a.gauge = bidx == 1 ? B.gauge : -1;
+    a.accumulate = accumulate;
+    float* p0 = (float*)L.plan[0];
+    float* p1 = (float*)L.plan[1];
+    {
+        Timed t(PK_LP_C2P);
+        a.Afrag = f32 ? B.lpf_fwd : B.lph_fwd; a.X = nullptr; a.Y = p0; a.ncols = cplx_cols;
+        HIPCHK(launch_proj_lp(NP, scheme, PROJ_C2P, a, L.stream));
+    }
+    {
+        Timed t(PK_LP_P2P_DIV);
+        a.X = p0; a.Y = p1; a.ncols = planes * NP;
+        HIPCHK(launch_proj_lp(NP, scheme, PROJ_P2P_DIV, a, L.stream));
+    }
+    if (bidx == 1) {
+        GaugeArgs ga{};
+        ga.G = p1; ga.f32 = 1; ga.prow = B.prow; ga.pdiag = B.pdiag; ga.wanted = wanted;
+        ga.relative = accumulate;
+        ga.gauge = B.gauge; ga.NP = NP; ga.g = g;
+        GaugeList gl{};
+        for (int f = 0; f < g.nf; ++f)
+            for (int k = 0; k < g.nmodes; ++k) {
+                const double xl = xlambda_of(g.alambda[f], g.alp, g.mode0 + k);
+                if (!(std::fabs(xl) <= 1.0e-10)) continue;
+                const int fm = f * g.nmodes + k;
+                for (int part = 0; part < 2; ++part) {
+                    if (part == 1 && g.mode0 + k == 0) continue;
+                    gl.plane[gl.count++] = 2 * fm + part;
+                    if (gl.count == kMaxGaugePlanes) {
+                        Timed t(PK_GAUGE);
+                        HIPCHK(launch_gauge(ga, gl, L.stream));
+                        gl.count = 0;
+                    }
+                }
+            }
+        if (gl.count) {
+            Timed t(PK_GAUGE);
+            HIPCHK(launch_gauge(ga, gl, L.stream));
+        }
+    }
+    {
+        Timed t(PK_LP_P2P);
+        a.Afrag = f32 ? B.lpf_bwd : B.lph_bwd; a.X = p1; a.Y = p0; a.ncols = planes * NP;
+        HIPCHK(launch_proj_lp(NP, scheme, PROJ_P2P, a, L.stream));
+    }
+    {
+        Timed t(PK_LP_P2C);
+        a.X = p0; a.Y = nullptr; a.ncols = cplx_cols;
+        HIPCHK(launch_proj_lp(NP, scheme, PROJ_P2C, a, L.stream));
+    }
+    return OK;
+}
+
+// Mixed-precision refined solve of a batch:
+//   u = S0(f);  repeat niter times: r = f - L u (FP64), u += S1(r);  boundary (FP64).
+// stats (host, may be null): [(niter+1) x nf x 2] = max|r_it|, max|f| per field,
+// including the residual after the last correction.
+int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* sol, const char* bounds,
+                       const double* alambda, int m, int nmodes, int ib, double alp, int scheme0, int scheme1,
+                       int niter, double* stats)
+{
+    if (nf < 1 || nf > kMaxFields) return fail(ERR_ARG, "nfields=%d outside [1, %d]", nf, kMaxFields);
+    int st = check_common(m, nmodes);
+    if (st) return st;
+    for (int sc : {scheme0, scheme1})
+        if (sc < LP_F32 || sc > LP_BF16X1) return fail(ERR_ARG, "precision scheme %d is not one of 1..4", sc);
+    if (niter < 0 || niter > 16) return fail(ERR_ARG, "niter=%d outside [0, 16]", niter);
+    const size_t field_bytes = (size_t)nmodes * m * m * sizeof(d2v);
+    if ((st = ensure_buf(L.rfield, L.rfield_bytes, field_bytes * nf))) return st;
+    const int nstat = stats ? (niter + 1) : 0;
+    if (nstat) {
+        if ((st = ensure_buf(L.stats, L.stats_bytes, sizeof(unsigned long long) * 2 * nf * nstat))) return st;
+        HIPCHK(hipMemsetAsync(L.stats, 0, sizeof(unsigned long long) * 2 * nf * nstat, L.stream));
+    }
+    for (int bidx = 0; bidx < 2; ++bidx) {
+        FieldGroup g{};
+        g.nmodes = nmodes; g.mode0 = ib; g.m = m; g.n = m - 2; g.alp = alp;
+        int fidx[kMaxFields];
+        for (int f = 0; f < nf; ++f) {
+            const int bi = bound_index(bounds[f]);
+            if (bi < 0) return fail(ERR_ARG, "bound '%c' of field %d is neither 'd' nor 'n'", bounds[f], f);
+            if (bi != bidx) continue;
+            if (!rhs[f] || !sol[f]) return fail(ERR_ARG, "null device pointer for field %d", f);
+            fidx[g.nf] = f;
+            g.in[g.nf] = (const d2v*)rhs[f];
+            g.out[g.nf] = (d2v*)sol[f];
+            g.alambda[g.nf] = alambda[f];
+            ++g.nf;
+        }
+        if (g.nf == 0) continue;
+        DevBasis& B = L.basis[bidx];
+        if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
+        if (g.m != B.m) return fail(ERR_ARG, "m=%d but the basis was set up for m=%d", g.m, B.m);
+        if ((st = run_group_lp(g, bidx, scheme0, 0, B.wanted))) return st;
+        ResidArgs ra{};
+        ra.g = g;
+        ra.Dfrag = B.opfrag;
+        ra.NP = B.NP;
+        for (int i = 0; i < g.nf; ++i) ra.r[i] = L.rfield + (size_t)i * nmodes * m * m;
+        FieldGroup gc = g;   // correction: right-hand side = residual, accumulated into sol
+        for (int i = 0; i < g.nf; ++i) gc.in[i] = ra.r[i];
+        for (int it = 0; it <= niter; ++it) {
+            if (it == niter && !nstat) break;   // the last residual is only for the report
+            {
+                Timed t(PK_RESID);
+                ra.stats = nstat ? L.stats + (size_t)2 * nf * it : nullptr;
+                for (int i = 0; i < g.nf; ++i) ra.slot[i] = fidx[i];
+                ra.neumann = bidx == 1;
+                HIPCHK(launch_resid(ra, L.stream));
+            }
+            if (bidx == 1) {   // drop the null-space part of the singular planes' residual
+                NullProjArgs np{};
+                np.q = B.qrow; np.p = B.pcol; np.stats = ra.stats;
+                np.nmodes = g.nmodes; np.m = g.m; np.n = g.n;
+                for (int i = 0; i < g.nf; ++i) { np.r[i] = ra.r[i]; np.slot[i] = fidx[i]; }
+                GaugeList gl{};
+                for (int f = 0; f < g.nf; ++f)
+                    for (int k = 0; k < g.nmodes; ++k) {
+                        if (!(std::fabs(xlambda_of(g.alambda[f], g.alp, g.mode0 + k)) <= 1.0e-10)) continue;
+                        for (int part = 0; part < 2; ++part) {
+                            if (part == 1 && g.mode0 + k == 0) continue;
+                            gl.plane[gl.count++] = 2 * (f * g.nmodes + k) + part;
+                            if (gl.count == kMaxGaugePlanes) {
+                                Timed t(PK_GAUGE);
+                                HIPCHK(launch_nullproj(np, gl, L.stream));
+                                gl.count = 0;
+                            }
+                        }
+                    }
+                if (gl.count) {
+                    Timed t(PK_GAUGE);
+                    HIPCHK(launch_nullproj(np, gl, L.stream));
+                }
+            }
+            if (it == niter) break;
+            if ((st = run_group_lp(gc, bidx, scheme1, 1, B.wanted))) return st;
+        }
+        {
+            Timed t(PK_BOUNDARY);
+            BoundaryArgs ba{};
+            ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
+            ba.mode = bidx == 1 ? BND_NEUMANN : BND_DIR_ZERO;
+            HIPCHK(launch_boundary(ba, L.stream));
+        }
+    }
+    if (nstat) {
+        std::vector<unsigned long long> h((size_t)2 * nf * nstat);
+        HIPCHK(hipMemcpyAsync(h.data(), L.stats, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              L.stream));
+        HIPCHK(hipStreamSynchronize(L.stream));
+        for (size_t e = 0; e < h.size(); ++e) stats[e] = __builtin_bit_cast(double, h[e]);
     }
     return OK;
 }
@@ -286,14 +484,12 @@ int duct_shutdown_(void)
 {
     if (!L.init) return OK;
     (void)hipStreamSynchronize(L.stream);
-    for (auto& B : L.basis) {
-        for (double** p : {&B.Afwd, &B.Abwd, &B.lam, &B.DC, &B.AM1, &B.D2b, &B.prow})
-            if (*p) { (void)hipFree(*p); *p = nullptr; }
-        B = DevBasis{};
-    }
-    for (auto& p : L.plan) if (p) { (void)hipFree(p); p = nullptr; }
-    for (auto& p : L.hfield) if (p) { (void)hipFree(p); p = nullptr; }
-    L.plan_bytes = L.hfield_bytes = 0;
+    for (auto& B : L.basis) free_basis(B);
+    for (auto& p : L.plan) free_ptr(p);
+    for (auto& p : L.hfield) free_ptr(p);
+    free_ptr(L.rfield);
+    free_ptr(L.stats);
+    L.plan_bytes = L.hfield_bytes = L.rfield_bytes = L.stats_bytes = 0;
     for (auto& r : L.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto& e : L.pool) (void)hipEventDestroy(e);
     L.recs.clear();
@@ -346,9 +542,7 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
 
     DevBasis& B = L.basis[bidx];
     HIPCHK(hipStreamSynchronize(L.stream));
-    for (double** p : {&B.Afwd, &B.Abwd, &B.lam, &B.DC, &B.AM1, &B.D2b, &B.prow})
-        if (*p) { HIPCHK(hipFree(*p)); *p = nullptr; }
-    B = DevBasis{};
+    free_basis(B);
 
     std::vector<double> fwd((size_t)NP * NP), bwd((size_t)NP * NP), lp(NP, 0.0), d2b((size_t)m * 2, 0.0),
         prow(NP, 0.0), am1(16, 0.0);
@@ -384,6 +578,58 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
     if ((st = up(B.AM1, am1.data(), 16))) return st;
     if ((st = up(B.D2b, d2b.data(), d2b.size()))) return st;
     if ((st = up(B.prow, prow.data(), prow.size()))) return st;
+
+    // mixed-precision solve: low-precision bases and the FP64 interior operator
+    {
+        std::vector<double> D2((size_t)m * m, 0.0), op((size_t)n * n, 0.0);
+        for (int j = 0; j < m; ++j)          // D2 = DC * DC (pseudomat.f90:110)
+            for (int k = 0; k < m; ++k) {
+                const double b = DC[k + (size_t)j * m];
+                for (int i = 0; i < m; ++i) D2[i + (size_t)j * m] += DC[i + (size_t)k * m] * b;
+            }
+        const int N = m - 1;
+        const double Det = DC[0] * DC[N + (size_t)N * m] - DC[(size_t)N * m] * DC[N];
+        for (int jj = 0; jj < n; ++jj)
+            for (int ii = 0; ii < n; ++ii) {
+                const int i = ii + 1, j = jj + 1;
+                double v = D2[i + (size_t)j * m];               // Dirichlet: pseudomat.f90:232-240
+                if (bidx == 1) {                                 // Neumann: :114-128
+                    const double ter1 = D2[i] * (DC[(size_t)N * m] * DC[N + (size_t)j * m] -
+                                                 DC[N + (size_t)N * m] * DC[(size_t)j * m]) / Det;
+                    const double ter3 = D2[i + (size_t)N * m] * (DC[N] * DC[(size_t)j * m] -
+                                                                 DC[0] * DC[N + (size_t)j * m]) / Det;
+                    v = ter1 + v + ter3;
+                }
+                op[ii + (size_t)jj * n] = v;
+            }
+        std::vector<double> opf((size_t)NP * NP);
+        build_frag(op.data(), n, n, NP, opf.data());
+        if ((st = up(B.opfrag, opf.data(), opf.size()))) return st;
+        if (bidx == 1) {
+            const int io = *iofzero - 1;
+            std::vector<double> q(n), pc(n);
+            for (int k = 0; k < n; ++k) {
+                q[k] = Pinv[io + (size_t)k * n];
+                pc[k] = P[k + (size_t)io * n];
+            }
+            if ((st = up(B.qrow, q.data(), q.size()))) return st;
+            if ((st = up(B.pcol, pc.data(), pc.size()))) return st;
+        }
+        auto upraw = [&](void*& dst, const void* src, size_t bytes) -> int {
+            HIPCHK(hipMalloc(&dst, bytes));
+            HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+            return OK;
+        };
+        std::vector<unsigned char> buf(lp_frag_bytes(NP, LP_BF16X6));
+        build_frag_lp(Pinv, n, n, NP, LP_F32, buf.data());
+        if ((st = upraw(B.lpf_fwd, buf.data(), lp_frag_bytes(NP, LP_F32)))) return st;
+        build_frag_lp(P, n, n, NP, LP_F32, buf.data());
+        if ((st = upraw(B.lpf_bwd, buf.data(), lp_frag_bytes(NP, LP_F32)))) return st;
+        build_frag_lp(Pinv, n, n, NP, LP_BF16X6, buf.data());
+        if ((st = upraw(B.lph_fwd, buf.data(), buf.size()))) return st;
+        build_frag_lp(P, n, n, NP, LP_BF16X6, buf.data());
+        if ((st = upraw(B.lph_bwd, buf.data(), buf.size()))) return st;
+    }
     B.m = m;
     B.n = n;
     B.NP = NP;
@@ -602,6 +848,22 @@ int duct_mat_invdiag_dev_(const int* Nx, const int* Ny, const double* xlam, cons
     Timed t(PK_INVDIAG);
     HIPCHK(launch_invdiag((const double*)*F, (const double*)*ED2, (double*)*C, (long)(*Nx) * (*Ny), *xlam, L.stream));
     return OK;
+}
+
+int duct_helm3d_mixed_batch_dev_(const int* nfields, const duct_devptr_t* rhs, const duct_devptr_t* sol,
+                                 const char* bounds, const double* alambda, const int* m, const int* nmodes,
+                                 const int* ib, const double* alp, const int* scheme0, const int* scheme1,
+                                 const int* niter, double* stats, size_t bounds_len)
+{
+    if (!nfields || !rhs || !sol || !bounds || !alambda || !m || !nmodes || !ib || !alp || !scheme0 || !scheme1 ||
+        !niter)
+        return fail(ERR_ARG, "null argument to duct_helm3d_mixed_batch_dev_");
+    if (bounds_len && (size_t)*nfields > bounds_len)
+        return fail(ERR_ARG, "bounds has %zu characters for %d fields", bounds_len, *nfields);
+    int st = ensure_init();
+    if (st) return st;
+    return solve_fields_mixed(*nfields, rhs, sol, bounds, alambda, *m, *nmodes, *ib, *alp, *scheme0, *scheme1,
+                              *niter, stats);
 }
 
 }  // extern "C"

@@ -33,19 +33,6 @@
 
 namespace duct {
 
-template <int NP>
-struct ProjShape {
-    static constexpr int WR = NP < 64 ? NP : 64;             // rows per wave
-    static constexpr int WM = (NP / WR) > 4 ? 4 : NP / WR;   // waves along rows
-    static constexpr int MT = WR * WM;                       // rows per workgroup
-    static constexpr int WN = 4 / WM;                        // waves along columns
-    static constexpr int NT = 64 * WN;                       // (virtual) columns per workgroup
-    static constexpr int TM = WR / 16;                       // 16-row tiles per wave
-    static constexpr int KT = 16;                            // k per LDS stage
-    static constexpr int LDB = KT + 1;                       // LDS column stride (doubles)
-    static constexpr int NK = NP / KT;                       // LDS stages
-    static constexpr int ROWBLOCKS = NP / MT;
-};
 
 // Virtual column v of a complex tile -> (complex column within tile, part).
 // Wave wn owns virtual columns [64 wn, 64 wn + 64) = 4 tiles of 16: tiles 0,1 are

@@ -1,0 +1,194 @@
+// FP64 collocation residual of the mixed-precision solve:
+//
+//   r = f - (D2B_y u + u D2B_z^T - xlambda u)          on the interior,
+//
+// D2B being the interior second-derivative operator the eigenbasis diagonalises
+// (pseudomat.f90:232-240 Dirichlet, :114-128 Neumann with the boundary rows
+// eliminated), so r is the residual of exactly the system solvediag solves.
+//
+// Both directional products accumulate into ONE register tile per wave with
+// MFMA row index z and column index y (so that lanes run along the contiguous
+// y of the field):
+//   z term:  sum_k D2B[z][k] u(y, k)   A = D2B fragments,  B = u tile [y][k]
+//   y term:  sum_k u(k, z) D2B[y][k]   A = u tile [z][k],  B = D2B fragments
+// The D2B fragment array (build_frag layout) serves as the A operand of one
+// and the B operand of the other.  A 256-thread workgroup owns a 64 z x 64 y
+// block of one complex plane (re and im); the two u slabs it streams are
+// staged through LDS, double buffered.  Workgroups are numbered so that all
+// blocks of a plane land on the same XCD (and its L2).
+#include "common.h"
+#include "proj.h"
+#include "proj_lp.h"
+
+namespace duct {
+
+namespace {
+constexpr int RB = 64;          // block edge (z and y)
+constexpr int RKT = 16;         // k per LDS stage
+constexpr int RLD = RKT + 1;    // LDS stride (doubles)
+constexpr int kXcd = 8;         // XCDs per MI355X
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* p, double v)
+{
+    // non-negative doubles order like their bit patterns
+    atomicMax(p, (unsigned long long)__builtin_bit_cast(unsigned long long, v));
+}
+}  // namespace
+
+__global__ __launch_bounds__(256, 2) void k_resid(ResidArgs a, int bpd, long nplanes)
+{
+    const FieldGroup& g = a.g;
+    const int NP = a.NP, n = g.n, m = g.m;
+    // XCD-aware numbering: 8 consecutive planes share a group of 8*bpd^2 ids,
+    // plane = group*8 + id%8, block = (id/8) % bpd^2.
+    const long bid = blockIdx.x;
+    const int tpp = bpd * bpd;
+    const long grp = bid / ((long)kXcd * tpp);
+    const long plane = grp * kXcd + bid % kXcd;
+    const int blk = (int)((bid / kXcd) % tpp);
+    if (plane >= nplanes) return;
+    const int f = (int)(plane / g.nmodes), mode = (int)(plane % g.nmodes);
+    const int z0 = (blk / bpd) * RB, y0 = (blk % bpd) * RB;
+    const d2v* U = g.out[f] + (size_t)mode * m * m;   // current solution
+    const d2v* F = g.in[f] + (size_t)mode * m * m;    // right-hand side
+
+    __shared__ double Uz[2][2][RB * RLD];   // [buf][re/im][y][k]: u(y0+y, k0+k)
+    __shared__ double Uy[2][2][RB * RLD];   // [buf][re/im][z][k]: u(k0+k, z0+z)
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int wz = wave & 1, wy = wave >> 1;           // wave tile: 32 z x 32 y
+
+    // loaders: 4 complex values of each slab per thread per stage
+    //   Uz: thread -> k = tid/16, y = (tid%16)*4 .. +3   (y contiguous in memory)
+    //   Uy: thread -> z = tid/4,  k = (tid%4)*4 .. +3    (k = y' contiguous)
+    const int uz_k = tid >> 4, uz_y = (tid & 15) * 4;
+    const int uy_z = tid >> 2, uy_k = (tid & 3) * 4;
+    d2v rz[4], ry[4];
+    auto load = [&](int ks) {
+        const int k0 = ks * RKT;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int y = y0 + uz_y + e, kz = k0 + uz_k;
+            rz[e] = (y < n && kz < n) ? U[(size_t)(y + 1) + (size_t)(kz + 1) * m] : d2v{0.0, 0.0};
+            const int z = z0 + uy_z, ky = k0 + uy_k + e;
+            ry[e] = (z < n && ky < n) ? U[(size_t)(ky + 1) + (size_t)(z + 1) * m] : d2v{0.0, 0.0};
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            Uz[buf][0][(uz_y + e) * RLD + uz_k] = rz[e][0];
+            Uz[buf][1][(uz_y + e) * RLD + uz_k] = rz[e][1];
+            Uy[buf][0][uy_z * RLD + uy_k + e] = ry[e][0];
+            Uy[buf][1][uy_z * RLD + uy_k + e] = ry[e][1];
+        }
+    };
+
+    // D2B fragments: row tile t (16 rows), k-pair kp: 64 lanes x 2 doubles
+    const d2v* Df = (const d2v*)a.Dfrag;
+    auto dfrag = [&](int rowtile, int kp) -> d2v {
+        if (rowtile * 16 >= NP) return d2v{0.0, 0.0};
+        return Df[((size_t)rowtile * (NP / 8) + kp) * 64 + lane];
+    };
+
+    d4v acc[2][2][2];   // [zt][yt][re/im]
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) acc[i][j][p] = d4v{0.0, 0.0, 0.0, 0.0};
+
+    const int zt0 = (z0 + wz * 32) / 16, yt0 = (y0 + wy * 32) / 16;   // global 16-row tiles
+    const int NK = (n + RKT - 1) / RKT;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int ks = 0; ks < NK; ++ks) {
+        const int buf = ks & 1;
+        if (ks + 1 < NK) load(ks + 1);
+        d2v az[2][2], by[2][2];   // D2B fragments for the z rows (A) and y rows (B), 2 k-pairs
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                az[t][h] = dfrag(zt0 + t, ks * 2 + h);
+                by[t][h] = dfrag(yt0 + t, ks * 2 + h);
+            }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int kl = kk * 4 + l4;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                double bu[2], au[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    bu[t] = Uz[buf][p][(wy * 32 + t * 16 + l16) * RLD + kl];   // B[k][j=y] = u(y, k)
+                    au[t] = Uy[buf][p][(wz * 32 + t * 16 + l16) * RLD + kl];   // A[i=z][k] = u(k, z)
+                }
+#pragma unroll
+                for (int zt = 0; zt < 2; ++zt)
+#pragma unroll
+                    for (int yt = 0; yt < 2; ++yt) {
+                        acc[zt][yt][p] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                            az[zt][kk >> 1][kk & 1], bu[yt], acc[zt][yt][p], 0, 0, 0);
+                        acc[zt][yt][p] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                            au[zt], by[yt][kk >> 1][kk & 1], acc[zt][yt][p], 0, 0, 0);
+                    }
+            }
+        }
+        if (ks + 1 < NK) store(buf ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue: D[i=z][j=y] in lane j + 16 (i%4), element i/4
+    const double xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
+    const bool zim = (g.mode0 + mode) == 0;
+    const bool singular = a.neumann && fabs(xlam) <= 1.0e-10;
+    d2v* R = a.r[f] + (size_t)mode * m * m;
+    double rmx = 0.0, fmx = 0.0;
+#pragma unroll
+    for (int zt = 0; zt < 2; ++zt)
+#pragma unroll
+        for (int yt = 0; yt < 2; ++yt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int z = z0 + wz * 32 + zt * 16 + 4 * e + l4;
+                const int y = y0 + wy * 32 + yt * 16 + l16;
+                if (z < n && y < n) {
+                    const size_t idx = (size_t)(y + 1) + (size_t)(z + 1) * m;
+                    const d2v u = U[idx], fv = F[idx];
+                    d2v lu = {acc[zt][yt][0][e], acc[zt][yt][1][e]};
+                    d2v rv = fv - (lu - xlam * u);
+                    if (zim) rv[1] = 0.0;
+                    R[idx] = rv;
+                    if (!singular) rmx = fmax(rmx, fmax(fabs(rv[0]), fabs(rv[1])));
+                    fmx = fmax(fmx, fmax(fabs(fv[0]), zim ? 0.0 : fabs(fv[1])));
+                }
+            }
+    if (a.stats) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            rmx = fmax(rmx, __shfl_xor(rmx, off, 64));
+            fmx = fmax(fmx, __shfl_xor(fmx, off, 64));
+        }
+        if (lane == 0) {
+            atomic_max_pos(a.stats + 2 * a.slot[f], rmx);
+            atomic_max_pos(a.stats + 2 * a.slot[f] + 1, fmx);
+        }
+    }
+}
+
+hipError_t launch_resid(const ResidArgs& a, hipStream_t st)
+{
+    const int bpd = (a.g.n + RB - 1) / RB;
+    const long nplanes = (long)a.g.nf * a.g.nmodes;
+    const long groups = (nplanes + kXcd - 1) / kXcd;
+    const long blocks = groups * kXcd * bpd * bpd;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_resid, dim3((unsigned)blocks), dim3(256), 0, st, a, bpd, nplanes);
+    return hipGetLastError();
+}
+
+}  // namespace duct

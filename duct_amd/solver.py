@@ -163,3 +163,36 @@ class KernelTimer:
         self.count = {k: cnt[i] for i, k in enumerate(_lib.PROF_CLASSES)}
         _lib.call("duct_prof_enable_", _lib.byref_int(0))
         return False
+
+
+SCHEMES = {"f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16x1": 4}
+
+
+def helm3d_mixed_batch(rhs_list, sol_list, bounds: str, alambdas, alp: float = 1.0, ib: int = 0,
+                       scheme0: str = "bf16x3", scheme1: str = "bf16x6", niter: int = 1, stats: bool = False):
+    """Mixed-precision refined solve of several fields (duct_helm3d_mixed_batch_dev_).
+    Returns None, or with stats=True a numpy array (niter+1, nfields, 2) of
+    [max|r_it|, max|f|] per refinement step (the last after the final correction)."""
+    import numpy as np
+    nf = len(rhs_list)
+    if nf != len(sol_list) or nf != len(bounds) or nf != len(alambdas):
+        raise ValueError("rhs, sol, bounds, alambdas lengths differ")
+    if scheme0 not in SCHEMES or scheme1 not in SCHEMES:
+        raise ValueError("scheme must be one of %s" % sorted(SCHEMES))
+    m = rhs_list[0].shape[-1]
+    for t in list(rhs_list) + list(sol_list):
+        _check_field(t, m)
+        if t.shape != rhs_list[0].shape:
+            raise ValueError("all fields must have the same shape")
+    for b in bounds:
+        _bound_bytes(b)
+    _use_current_stream()
+    R = (ctypes.c_size_t * nf)(*[t.data_ptr() for t in rhs_list])
+    S = (ctypes.c_size_t * nf)(*[t.data_ptr() for t in sol_list])
+    A = (ctypes.c_double * nf)(*[float(a) for a in alambdas])
+    st = np.zeros((niter + 1, nf, 2)) if stats else None
+    _lib.call("duct_helm3d_mixed_batch_dev_", _lib.byref_int(nf), R, S, bounds.encode(), A, _lib.byref_int(m),
+              _lib.byref_int(rhs_list[0].shape[0]), _lib.byref_int(ib), _lib.byref_double(alp),
+              _lib.byref_int(SCHEMES[scheme0]), _lib.byref_int(SCHEMES[scheme1]), _lib.byref_int(niter),
+              st.ctypes.data if stats else None, ctypes.c_size_t(nf))
+    return st

@@ -112,6 +112,24 @@ int duct_int2d3d_(double *sol, const double *rhs, const char *bound, const doubl
  * callers such as precomp_mass_flow (src/operators.F:2276).               */
 int duct_solvediag_(double *U, const char *bound, const double *xlambda, size_t bound_len);
 
+/* Mixed-precision refined variant of duct_helm3d_batch_dev_ (same fields,
+ * boundary treatment and results to within the refinement tolerance).  The
+ * reference solves in FP64 only (solvediag.f90:51-80, :140-219); here the four
+ * eigenbasis products run on the fast matrix cores and FP64 is kept for the
+ * residual r = f - (D2B_y u + u D2B_z^T - xlambda u), the eigenvalue divide,
+ * the Neumann gauge, edges and corners:
+ *   u = S[scheme0](f);  niter times: r = f - L u,  u += S[scheme1](r).
+ * scheme: 1 = FP32 MFMA, 2 = bf16 split in 2 terms (3 products), 3 = bf16
+ * split in 3 terms (6 products), 4 = plain bf16.  niter in [0, 16].
+ * stats (host, may be NULL): (niter+1) x nfields x 2 doubles = max|r| after
+ * each step (the last one after the final correction) and max|f|.           */
+int duct_helm3d_mixed_batch_dev_(const int *nfields, const duct_devptr_t *rhs,
+                                 const duct_devptr_t *sol, const char *bounds,
+                                 const double *alambda, const int *m, const int *nmodes,
+                                 const int *ib, const double *alp, const int *scheme0,
+                                 const int *scheme1, const int *niter, double *stats,
+                                 size_t bounds_len);
+
 /* ---- bandwidth-bound helpers (reference GPU glue) ----------------------- */
 
 /* C = alpha * A^T for nbatch consecutive Nx x Ny FP64 matrices (column-major,
@@ -142,7 +160,8 @@ int duct_prof_enable_(const int *on);
 /* Synchronise and return, for each kernel class k < *nclass, the summed
  * device time in ms and the launch count.  Classes: 0 proj_c2p,
  * 1 proj_p2p_div, 2 gauge, 3 proj_p2p, 4 proj_p2c, 5 boundary, 6 bc_correct,
- * 7 transpose, 8 invdiag. */
+ * 7 transpose, 8 invdiag, 9 lp_c2p, 10 lp_p2p_div, 11 lp_p2p, 12 lp_p2c,
+ * 13 resid. */
 int duct_prof_read_(const int *nclass, double *ms, int *count);
 
 #ifdef __cplusplus

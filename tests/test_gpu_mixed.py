@@ -1,0 +1,151 @@
+"""GPU tests of the mixed-precision refined solve against the FP64 oracle.
+
+Bar: relative error max|u_mixed - u_oracle| / max|u_oracle| <= 1e-10 per field
+(the tolerance the task states for the refined solve), for every precision
+scheme with the refinement count it needs; the FP64 residual kernel is checked
+against a numpy FP64 residual of the same iterate.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    import duct_amd
+    torch.cuda.set_device(0)
+    duct_amd.init(0)
+    yield duct_amd
+    duct_amd.shutdown()
+
+
+_cur = {"m": None}
+
+
+def ensure(dev, m):
+    if _cur["m"] != m:
+        dev.setup(m)
+        _cur["m"] = m
+    return dev.solver._state["setup"]
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def run_mixed(dev, fields, bounds, alams, **kw):
+    import torch
+    rhs = [torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in fields]
+    sol = [torch.zeros_like(r) for r in rhs]
+    st = dev.helm3d_mixed_batch(rhs, sol, bounds, alams, stats=True, **kw)
+    torch.cuda.synchronize()
+    return [s.cpu().numpy() for s in sol], st
+
+
+# Refinement contracts the error by roughly eps(scheme1) * cond(L); cond(L) grows
+# like m^4 and is largest for the alpha = 0 Poisson field, so the number of
+# corrections each scheme needs grows with m (measured: at m = 129 one bf16x3
+# correction leaves ~1e-9 on the Poisson field, two leave ~1e-11).
+SCHEMES = {
+    33: [("bf16x3", "bf16x6", 1), ("bf16x3", "bf16x3", 2), ("f32", "f32", 1), ("bf16x6", "bf16x3", 1),
+         ("bf16x6", "bf16x6", 1)],
+    129: [("bf16x3", "bf16x6", 2), ("bf16x3", "bf16x3", 2), ("f32", "f32", 1), ("bf16x6", "bf16x6", 1)],
+}
+CASES = [(m, M, s0, s1, it) for m, M in ((33, 16), (129, 8)) for s0, s1, it in SCHEMES[m]]
+
+
+@pytest.mark.parametrize("m,M,s0,s1,niter", CASES)
+def test_mixed_matches_fp64_oracle(dev, oracle_mod, m, M, s0, s1, niter):
+    from duct_amd import workloads
+    ensure(dev, m)
+    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(m))
+    cfg = workloads.CONFIGS[2]
+    fields = [workloads.rhs_field(cfg, f, m=m, nmodes=M) for f in range(4)]
+    alams = [workloads.alambda_of(cfg, f) for f in range(4)]
+    got, st = run_mixed(dev, fields, workloads.BOUNDS, alams, scheme0=s0, scheme1=s1, niter=niter)
+    oracle_mod.use_blas(True)
+    try:
+        for f in range(4):
+            ref, _ = oracle_mod.helm3d(ob, fields[f], workloads.BOUNDS[f], alams[f], nthreads=8)
+            e = relerr(got[f], ref)
+            print("m=%d %s->%s x%d field %s: rel.err %.2e, residual history %s" % (
+                m, s0, s1, niter, workloads.FIELDS[f], e, " ".join("%.1e" % (r / fm) for r, fm in st[:, f])),
+                flush=True)
+            assert e <= TOL
+            assert not got[f][0].imag.any()
+    finally:
+        oracle_mod.use_blas(False)
+    # refinement reduces the residual (for the Poisson field's k=0 plane the
+    # residual is measured after removing the part of f in the null space of L)
+    assert np.all(st[-1, :, 0] < 1e-4 * st[0, :, 0])
+
+
+def test_residual_kernel_matches_numpy(dev):
+    """niter=0: stats[0] is max|f - L u0| of the returned u0, computed by the FP64 MFMA kernel."""
+    import torch
+    from duct_amd import workloads
+    m, M = 65, 5
+    s = ensure(dev, m)
+    cfg = workloads.CONFIGS[2]
+    fields = [workloads.rhs_field(cfg, f, m=m, nmodes=M) for f in (0, 3)]
+    alams = [1000.0, 0.0]
+    got, st = run_mixed(dev, fields, "dn", alams, scheme0="bf16x3", scheme1="bf16x3", niter=0)
+    for i, op in enumerate((s.dirichlet.op, s.neumann.op)):
+        u = got[i][:, 1:-1, 1:-1]
+        f = fields[i][:, 1:-1, 1:-1]
+        k = np.arange(M)
+        xl = alams[i] + k.astype(float) ** 2
+        r = f - (u @ op.T + np.einsum("ij,kjl->kil", op, u) - xl[:, None, None] * u)
+        r[0].imag = 0.0
+        if i == 1:   # the singular k=0 Poisson plane is reported after removing its null-space part
+            b = s.neumann
+            q, p = b.Pinv[b.iofzero - 1, :], b.P[:, b.iofzero - 1]
+            r0 = r[0].real
+            r[0] = r0 - (q @ r0 @ q) * np.outer(p, p)
+        def cmax(a):   # the kernel reports the largest real or imaginary component
+            return max(np.max(np.abs(a.real)), np.max(np.abs(a.imag)))
+        want = cmax(r)
+        scale = np.abs(op).sum(1).max() * np.max(np.abs(u)) * 1e-13
+        assert abs(st[0, i, 0] - want) <= scale + 1e-9 * want, (i, st[0, i, 0], want)
+        f0 = f.copy()
+        f0[0] = f0[0].real
+        assert st[0, i, 1] == cmax(f0)
+
+
+def test_mixed_rejects_bad_scheme(dev):
+    import torch
+    from duct_amd._lib import DuctError, call, byref_int, byref_double
+    import ctypes
+    ensure(dev, 33)
+    x = torch.zeros((2, 33, 33), dtype=torch.complex128, device="cuda")
+    P = (ctypes.c_size_t * 1)(x.data_ptr())
+    A = (ctypes.c_double * 1)(1.0)
+    with pytest.raises(DuctError):
+        call("duct_helm3d_mixed_batch_dev_", byref_int(1), P, P, b"d", A, byref_int(33), byref_int(2),
+             byref_int(0), byref_double(1.0), byref_int(9), byref_int(2), byref_int(1), None, ctypes.c_size_t(1))
+
+
+def test_mixed_full_size_config2(dev, oracle_mod):
+    """257^2 x 256 modes x 4 fields, bf16x3 -> bf16x6, one refinement: <= 1e-10 on sampled modes."""
+    import torch
+    from duct_amd import workloads
+    cfg = workloads.CONFIGS[2]
+    ensure(dev, cfg.m)
+    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(cfg.m))
+    fields = [workloads.rhs_field(cfg, f) for f in range(4)]
+    alams = [workloads.alambda_of(cfg, f) for f in range(4)]
+    got, st = run_mixed(dev, fields, workloads.BOUNDS, alams, scheme0="bf16x3", scheme1="bf16x6", niter=1)
+    oracle_mod.use_blas(True)
+    try:
+        for f in range(4):
+            for k0 in (0, 1, 2, 100, 255):
+                ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
+                e = relerr(got[f][k0:k0 + 1], ref)
+                print("config 2 field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
+                assert e <= TOL
+    finally:
+        oracle_mod.use_blas(False)
