@@ -3,16 +3,25 @@
 Workload (N=1): 257 x 257 CGL points x 256 complex Fourier modes x 4 fields
 (u, v, w: Dirichlet Helmholtz, alambda = 1000; p: Neumann Poisson, alambda = 0,
 k=0 gauge), seeded synthetic right-hand sides, inputs resident in HBM.
-One step = one batched solve of all four fields (4 projection passes per
-boundary group + gauge + boundary kernels).
+One step = one batched solve of all four fields.
+
+Two paths are timed on the same inputs:
+  fp64   the plain FP64 form: four FP64-MFMA eigenbasis passes per boundary group
+  mixed  the refined form: low-precision passes (FP32 or split-bf16 MFMA),
+         FP64 residual, correction(s) accumulated into the FP64 solution
+The reported `value` is the faster path whose result is within 1e-10 (relative,
+max norm, every field) of the oracle; both paths' throughput, errors and the
+refinement count are in the line.
 
 Metric: mode-solves/s, a mode-solve being one (field, k_x) complex 2-D solve.
-Roofline: the projection kernel (k_proj, FP64 MFMA) against the FP64 MFMA peak,
-with the algorithmic 2 n^3 FLOP per real plane and pass (n = m-2, padding not
-counted, the imaginary plane at k=0 not counted), timed with hipEvents the
-library records around every launch on its stream.
-cpu_baseline: the oracle (C restatement of solvediag/int2d3d/helm3d with a
-BLAS dgemm per product) on the host cores over the same workload.
+Roofline: the chosen path's dominant kernel class, algorithmic FLOP per launch
+(2 n^3 per real plane and product for a projection pass, 4 n^3 for the dense
+FP64 residual; n = m-2, padding and the unsolved imaginary plane at k=0 not
+counted) over its mean launch time from hipEvents recorded on the library's
+stream during the timed steps.
+cpu_baseline: the oracle (C restatement of solvediag/int2d3d/helm3d, one BLAS
+dgemm per product) on the host cores over the whole workload; it also serves
+as the checker of both GPU results.
 """
 from __future__ import annotations
 
@@ -28,8 +37,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-FP64_MFMA_PEAK = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
-HBM_PEAK = 8.0e12          # bytes/s
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15}   # MI355X dense matrix peaks, FLOP/s
+TOL = 1e-10
 
 
 def parse():
@@ -38,37 +47,31 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--mixed", default="bf16x6,bf16x6,1",
+                    help="scheme0,scheme1,niter of the refined solve")
+    ap.add_argument("--path", choices=["auto", "fp64", "mixed"], default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
 
-def cpu_baseline(cfg, fields, alams, sols_gpu, threads):
-    """Oracle helm3d on the host cores over the whole workload; also the checker
-    of the GPU result (max relative error over all fields)."""
+def cpu_baseline(cfg, fields, alams, threads):
+    """The oracle on the host cores over the whole workload (also the checker)."""
     import oracle
+    from duct_amd import workloads
     blas = oracle.use_blas(True)
     ob = oracle.OracleBasis(oracle.pseudomat(cfg.m))
-    from duct_amd import workloads
     t0 = time.perf_counter()
-    worst = 0.0
-    refs = []
-    for f in range(cfg.nfields):
-        ref, _ = oracle.helm3d(ob, fields[f], workloads.BOUNDS[f], alams[f], alp=cfg.alp, ib=0, nthreads=threads)
-        refs.append(ref)
+    refs = [oracle.helm3d(ob, fields[f], workloads.BOUNDS[f], alams[f], alp=cfg.alp, ib=0, nthreads=threads)[0]
+            for f in range(cfg.nfields)]
     t_all = time.perf_counter() - t0
-    for f in range(cfg.nfields):
-        ref = refs[f]
-        worst = max(worst, float(np.max(np.abs(sols_gpu[f] - ref)) / np.max(np.abs(ref))))
-    # one core: a few modes of u, scaled
     k1 = 4
     t0 = time.perf_counter()
     oracle.helm3d(ob, fields[0][:k1], "d", alams[0], alp=cfg.alp, ib=1, nthreads=1)
     t_one = time.perf_counter() - t0
     oracle.use_blas(False)
-    nsolves = cfg.nfields * cfg.nmodes
-    return {
-        "value": nsolves / t_all,
+    out = {
+        "value": cfg.nfields * cfg.nmodes / t_all,
         "unit": "mode-solves/s",
         "cores": threads,
         "kind": "port",
@@ -76,8 +79,12 @@ def cpu_baseline(cfg, fields, alams, sols_gpu, threads):
                   "helm3d/int2d3d/solvediag, %s), OpenMP over modes; %.2f s" % (
                       cfg.nfields, cfg.nmodes, cfg.m, blas, t_all),
         "value_1core": k1 / t_one,
-        "gpu_max_rel_err_vs_oracle": worst,
     }
+    return out, refs
+
+
+def max_rel_err(sols, refs):
+    return max(float(np.max(np.abs(s - r)) / np.max(np.abs(r))) for s, r in zip(sols, refs))
 
 
 def main():
@@ -94,6 +101,8 @@ def main():
     cfg = workloads.CONFIGS[args.config]
     m, M, F = cfg.m, cfg.nmodes, cfg.nfields
     n = m - 2
+    s0, s1, niter = args.mixed.split(",")
+    niter = int(niter)
     torch.cuda.set_device(0)
     duct_amd.init(0)
     duct_amd.setup(m)
@@ -101,43 +110,75 @@ def main():
     alams = [workloads.alambda_of(cfg, f) for f in range(F)]
     bounds = workloads.BOUNDS[:F]
     rhs = [torch.from_numpy(x).cuda() for x in fields]
-    sol = [torch.zeros_like(r) for r in rhs]
+    sol = {"fp64": [torch.zeros_like(r) for r in rhs], "mixed": [torch.zeros_like(r) for r in rhs]}
+    steps = {
+        "fp64": lambda: duct_amd.helm3d_batch(rhs, sol["fp64"], bounds, alams, alp=cfg.alp, ib=0),
+        "mixed": lambda: duct_amd.helm3d_mixed_batch(rhs, sol["mixed"], bounds, alams, alp=cfg.alp, ib=0,
+                                                     scheme0=s0, scheme1=s1, niter=niter),
+    }
+    planes = F * (2 * M - 1)                      # Im at k=0 not solved
+    nprod = {"f32": 1, "bf16x1": 1, "bf16x3": 3, "bf16x6": 6}
 
-    def step():
-        duct_amd.helm3d_batch(rhs, sol, bounds, alams, alp=cfg.alp, ib=0)
+    def flop_per_step(kind):
+        """Algorithmic FLOP of one step of a kernel class."""
+        g = 2.0 * n ** 3 * planes                  # one projection pass, one product
+        if kind.startswith("proj_"):
+            return g
+        if kind.startswith("lp_"):
+            return g * (nprod[s0] + niter * nprod[s1])
+        if kind == "resid":
+            return 2.0 * g * niter                 # 4 n^3 per plane, one residual per correction
+        return 0.0
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    with duct_amd.KernelTimer() as kt:
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
+    res = {}
+    for path in ("fp64", "mixed"):
+        for _ in range(args.warmup):
+            steps[path]()
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-    dt = (t1 - t0) / args.steps
-    nsolves = F * M
-    value = nsolves / dt
+        with duct_amd.KernelTimer() as kt:
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                steps[path]()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
+        classes = {}
+        for k in kt.ms:
+            if not kt.count[k]:
+                continue
+            classes[k] = {"launches_per_step": kt.count[k] / args.steps, "avg_us": kt.ms[k] / kt.count[k] * 1e3,
+                          "ms_per_step": kt.ms[k] / args.steps}
+        res[path] = {"ms_per_step": dt * 1e3, "mode_solves_per_s": F * M / dt, "kernels": classes}
 
-    # roofline of the projection kernel (all four passes share k_proj)
-    planes = F * (2 * M - 1)                    # Im at k=0 not solved
-    flop_step = 8.0 * n ** 3 * planes           # 4 products x 2 n^3 per real plane
-    proj = ["proj_c2p", "proj_p2p_div", "proj_p2p", "proj_p2c"]
-    proj_ms = sum(kt.ms[k] for k in proj)
-    proj_launches = sum(kt.count[k] for k in proj)
-    avg_launch_s = proj_ms / 1e3 / proj_launches
-    flop_per_launch = flop_step * args.steps / proj_launches
-    achieved = flop_per_launch / avg_launch_s
-    kernels = {k: {"launches": kt.count[k], "avg_us": (kt.ms[k] / kt.count[k] * 1e3) if kt.count[k] else 0.0}
-               for k in kt.ms if kt.count[k]}
-    step_kernel_ms = sum(kt.ms.values()) / args.steps
-
-    sols_host = [s.cpu().numpy() for s in sol]
-    cpu = None
+    cpu, refs = None, None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(cfg, fields, alams, sols_host, threads)
+        cpu, refs = cpu_baseline(cfg, fields, alams, threads)
+        for path in res:
+            res[path]["max_rel_err_vs_oracle"] = max_rel_err([t.cpu().numpy() for t in sol[path]], refs)
 
+    ok = {p: (refs is None or res[p].get("max_rel_err_vs_oracle", 1.0) <= TOL) for p in res}
+    if args.path != "auto":
+        chosen = args.path
+    else:
+        cands = [p for p in res if ok[p]] or ["fp64"]
+        chosen = max(cands, key=lambda p: res[p]["mode_solves_per_s"])
+
+    # roofline of the chosen path's dominant kernel class
+    ks = res[chosen]["kernels"]
+    groups = {}
+    for k, v in ks.items():
+        key = "proj" if k.startswith("proj_") else "lp" if k.startswith("lp_") else k
+        groups.setdefault(key, []).append(k)
+    dom = max(groups, key=lambda g: sum(ks[k]["ms_per_step"] for k in groups[g]))
+    dom_ms = sum(ks[k]["ms_per_step"] for k in groups[dom])
+    dom_launches = sum(ks[k]["launches_per_step"] for k in groups[dom])
+    dom_flop = sum(flop_per_step(k) for k in groups[dom])
+    unit_peak = {"proj": "f64", "resid": "f64", "lp": "f32" if s0 == "f32" else "bf16"}.get(dom, "f64")
+    avg_launch_s = dom_ms / 1e3 / dom_launches if dom_launches else float("nan")
+    flop_per_launch = dom_flop / dom_launches if dom_launches else 0.0
+    achieved = flop_per_launch / avg_launch_s if dom_launches else 0.0
+
+    value = res[chosen]["mode_solves_per_s"]
     out = {
         "metric": "Helmholtz mode-solves/s at 257^2 x256 modes: mixed vs FP64 @1e-10 err; %MFMA peak",
         "value": value,
@@ -145,24 +186,31 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt * 1e3,
+        "ms_per_step": res[chosen]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if chosen == "fp64" else "%s+%s refined, f64 residual" % (s0, s1),
         "data": "synthetic (seeded Chebyshev-series right-hand sides, inputs resident in HBM)",
-        "config": {"workload": "cross-plane solve stage, plain FP64 path: %d^2 CGL x %d modes x %d fields "
-                               "(u,v,w Dirichlet alambda=1000; p Neumann alambda=0)" % (m, M, F),
-                   "m": m, "nmodes": M, "nfields": F, "precision_path": "fp64"},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK, "traffic": None,
-                     "kernel": "k_proj (FP64 MFMA projection pass, 4 launches per boundary group per step)",
+        "config": {"workload": "cross-plane solve stage: %d^2 CGL x %d modes x %d fields (u,v,w Dirichlet "
+                               "alambda=1000; p Neumann alambda=0)" % (m, M, F),
+                   "m": m, "nmodes": M, "nfields": F, "path": chosen,
+                   "mixed": {"scheme0": s0, "scheme1": s1, "refinements": niter}},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[unit_peak] / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK[unit_peak], "traffic": None,
+                     "kernel": {"proj": "k_proj (FP64 MFMA projection pass)",
+                                "lp": "k_proj_lp (%s MFMA projection pass)" % unit_peak,
+                                "resid": "k_resid (FP64 MFMA residual)"}.get(dom, dom),
                      "flop_per_launch": flop_per_launch, "avg_launch_us": avg_launch_s * 1e6},
-        "kernels": kernels,
-        "kernel_ms_per_step": step_kernel_ms,
-        "stage_tflops": flop_step / dt / 1e12,
+        "paths": res,
+        "mixed_vs_fp64": res["mixed"]["mode_solves_per_s"] / res["fp64"]["mode_solves_per_s"],
+        "fp64_mfma_frac": None,
         "cpu_baseline": cpu,
     }
+    kf = res["fp64"]["kernels"]
+    proj = [k for k in kf if k.startswith("proj_")]
+    t = sum(kf[k]["ms_per_step"] for k in proj) / 1e3
+    out["fp64_mfma_frac"] = (4 * 2.0 * n ** 3 * planes / t) / PEAK["f64"] if t else None
     print(json.dumps(out), flush=True)
     duct_amd.shutdown()
 
