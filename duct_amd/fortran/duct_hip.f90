@@ -1,0 +1,116 @@
+! Fortran binding of libduct_hip.so (include/duct_hip.h) for duct's Fortran
+! callers: pseudomat.f90 hands its eigenbases to duct_setup, and helm3d.f90 /
+! int2d3d.f90 / solvediag.f90 call duct_helm3d / duct_int2d3d / duct_solvediag
+! instead of their CPU loops or the cuBLAS path.  Every entry returns an
+! integer status; duct_check prints the library's message and stops, the way
+! the reference stops on a bad boundary character (solvediag.f90:324-326).
+module duct_hip
+  use iso_c_binding
+  implicit none
+  private
+  public :: duct_init, duct_shutdown, duct_setup, duct_helm3d, duct_helm3d_dev, &
+            duct_helm3d_batch_dev, duct_int2d3d, duct_solvediag, duct_check
+
+  interface
+    integer(c_int) function duct_init(device) bind(C, name="duct_init_")
+      import :: c_int
+      integer(c_int), intent(in) :: device
+    end function
+
+    integer(c_int) function duct_shutdown() bind(C, name="duct_shutdown_")
+      import :: c_int
+    end function
+
+    integer(c_int) function duct_last_error(buf, len) bind(C, name="duct_last_error_")
+      import :: c_int, c_char
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_int), intent(in) :: len
+    end function
+
+    ! pseudomat.f90:188-221 / :265-298 (after dgeev, dgetrf/dgetri)
+    integer(c_int) function duct_setup(bound, m, P, Pinv, lam, DC, AM1, iofzero, wanted, bound_len) &
+        bind(C, name="duct_setup_")
+      import :: c_int, c_double, c_char, c_size_t
+      character(kind=c_char), intent(in) :: bound
+      integer(c_int), intent(in) :: m, iofzero
+      real(c_double), intent(in) :: P(*), Pinv(*), lam(*), DC(*), AM1(*)
+      real(c_double), intent(in) :: wanted
+      integer(c_size_t), value :: bound_len
+    end function
+
+    ! helm3d.f90:2 on host arrays: rhs, sol complex(8) (my, mz, nmodes)
+    integer(c_int) function duct_helm3d(rhs, sol, m, nmodes, ib, alp, alambda, bound, bound_len) &
+        bind(C, name="duct_helm3d_")
+      import :: c_int, c_double, c_double_complex, c_char, c_size_t
+      complex(c_double_complex), intent(inout) :: rhs(*)
+      complex(c_double_complex), intent(out) :: sol(*)
+      integer(c_int), intent(in) :: m, nmodes, ib
+      real(c_double), intent(in) :: alp, alambda
+      character(kind=c_char), intent(in) :: bound
+      integer(c_size_t), value :: bound_len
+    end function
+
+    ! helm3d on device-resident fields (addresses held in integer(8), as the
+    ! reference's gpudevice module holds its devptr_t values, mod.f90:442-481)
+    integer(c_int) function duct_helm3d_dev(rhs, sol, m, nmodes, ib, alp, alambda, bound, bound_len) &
+        bind(C, name="duct_helm3d_dev_")
+      import :: c_int, c_double, c_size_t, c_char
+      integer(c_size_t), intent(in) :: rhs, sol
+      integer(c_int), intent(in) :: m, nmodes, ib
+      real(c_double), intent(in) :: alp, alambda
+      character(kind=c_char), intent(in) :: bound
+      integer(c_size_t), value :: bound_len
+    end function
+
+    integer(c_int) function duct_helm3d_batch_dev(nfields, rhs, sol, bounds, alambda, m, nmodes, ib, alp, &
+        bounds_len) bind(C, name="duct_helm3d_batch_dev_")
+      import :: c_int, c_double, c_size_t, c_char
+      integer(c_int), intent(in) :: nfields, m, nmodes, ib
+      integer(c_size_t), intent(in) :: rhs(*), sol(*)
+      character(kind=c_char), intent(in) :: bounds(*)
+      real(c_double), intent(in) :: alambda(*), alp
+      integer(c_size_t), value :: bounds_len
+    end function
+
+    ! int2d3d.f90:2: one complex plane
+    integer(c_int) function duct_int2d3d(sol, rhs, bound, xlambda, imode, bound_len) &
+        bind(C, name="duct_int2d3d_")
+      import :: c_int, c_double, c_double_complex, c_char, c_size_t
+      complex(c_double_complex), intent(out) :: sol(*)
+      complex(c_double_complex), intent(in) :: rhs(*)
+      character(kind=c_char), intent(in) :: bound
+      real(c_double), intent(in) :: xlambda
+      integer(c_int), intent(in) :: imode
+      integer(c_size_t), value :: bound_len
+    end function
+
+    ! solvediag.f90:2: one real plane U(my,mz), in place
+    integer(c_int) function duct_solvediag(U, bound, xlambda, bound_len) bind(C, name="duct_solvediag_")
+      import :: c_int, c_double, c_char, c_size_t
+      real(c_double), intent(inout) :: U(*)
+      character(kind=c_char), intent(in) :: bound
+      real(c_double), intent(in) :: xlambda
+      integer(c_size_t), value :: bound_len
+    end function
+  end interface
+
+contains
+
+  subroutine duct_check(status, where)
+    integer(c_int), intent(in) :: status
+    character(*), intent(in) :: where
+    character(kind=c_char) :: buf(512)
+    character(len=512) :: msg
+    integer :: i, st
+    if (status == 0) return
+    st = duct_last_error(buf, 512_c_int)
+    msg = ' '
+    do i = 1, 512
+      if (buf(i) == c_null_char) exit
+      msg(i:i) = buf(i)
+    end do
+    write(6,*) 'in ', where, ': libduct_hip status ', status, ': ', trim(msg)
+    stop
+  end subroutine duct_check
+
+end module duct_hip
