@@ -149,8 +149,9 @@ __global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
     d2v* in = const_cast<d2v*>(g.in[f]) + (size_t)mode * mm;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    if (a.mode == BND_DIR_ZERO || a.mode == BND_DIR_KEEP) {
-        const bool zero = a.mode == BND_DIR_ZERO;
+    if (a.mode == BND_DIR_ZERO || a.mode == BND_DIR_KEEP || a.mode == BND_OUT_ZERO) {
+        const bool zero = a.mode != BND_DIR_KEEP;
+        if (a.mode == BND_OUT_ZERO) in = out;   // leave the input field alone
         for (int e = tid; e < 4 * m; e += kAuxThreads) {
             const int side = e / m, t = e % m;
             size_t idx;
@@ -291,6 +292,114 @@ hipError_t launch_bc_correct(const BcCorrectArgs& a, hipStream_t st)
     if (planes <= 0 || elems <= 0) return hipSuccess;
     dim3 grid((unsigned)((elems + 255) / 256), planes);
     hipLaunchKernelGGL(k_bc_correct, grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace duct
+
+namespace duct {
+
+// ---------------------------------------------------------------------------
+// out_int = in_int + E^T in_boundary for one complex plane per workgroup.
+// With d0_i = DC(1,i), dm_i = DC(m,i) and the corner sums ff of the forward
+// (solvediag.f90:280-302), U_corner = AM1 ff gives ffbar = AM1^T Ubar_corner;
+// each edge value collects its own Ubar plus its share of ffbar; the edges are
+// U(i,1) = (-DC(m,m) sb_i + DC(1,m) st_i)/det, U(i,m) = (DC(m,1) sb_i - DC(1,1) st_i)/det
+// with sb_i = sum_j DC(1,j) U(i,j), st_i = sum_j DC(m,j) U(i,j) (and the same
+// along y), so
+//   out(i,j) = in(i,j) + DC(1,j) sbbar_i + DC(m,j) stbar_i + DC(1,i) slbar_j + DC(m,i) srbar_j.
+__global__ __launch_bounds__(kAuxThreads) void k_boundary_adj(BoundaryAdjArgs a)
+{
+    const FieldGroup& g = a.g;
+    const int fm = blockIdx.x;
+    const int f = fm / g.nmodes, mode = fm % g.nmodes;
+    const int m = g.m;
+    const size_t mm = (size_t)m * m;
+    const d2v* in = g.in[f] + (size_t)mode * mm;
+    d2v* out = g.out[f] + (size_t)mode * mm;
+    const int tid = threadIdx.x;
+    const double* DC = a.DC;
+    const double DC00 = DC[0], DCmm = DC[(m - 1) + (size_t)(m - 1) * m];
+    const double DC0m = DC[(size_t)(m - 1) * m], DCm0 = DC[m - 1];
+    const double det = DC00 * DCmm - DC0m * DCm0;
+
+    __shared__ d2v ffb[4];
+    __shared__ d2v sb[kMaxM], stt[kMaxM], sl[kMaxM], sr[kMaxM];
+    if (tid < 4) {   // ffbar(c) = sum_r AM1(r,c) Ubar_corner(r)
+        const size_t cidx[4] = {0, (size_t)(m - 1), (size_t)(m - 1) + (size_t)(m - 1) * m, (size_t)(m - 1) * m};
+        d2v v = {0.0, 0.0};
+        for (int r = 0; r < 4; ++r) v += a.AM1[r + 4 * tid] * in[cidx[r]];
+        ffb[tid] = v;
+    }
+    __syncthreads();
+    for (int i = 1 + tid; i < m - 1; i += kAuxThreads) {
+        const double d0 = DC[(size_t)i * m], dm = DC[(m - 1) + (size_t)i * m];
+        const d2v eb = in[(size_t)i] - d0 * ffb[0] + dm * ffb[1];
+        const d2v et = in[(size_t)i + (size_t)(m - 1) * m] - dm * ffb[2] - d0 * ffb[3];
+        const d2v el = in[(size_t)i * m] - d0 * ffb[0] + dm * ffb[3];
+        const d2v er = in[(m - 1) + (size_t)i * m] - d0 * ffb[1] - dm * ffb[2];
+        sb[i] = (-DCmm * eb + DCm0 * et) / det;
+        stt[i] = (DC0m * eb - DC00 * et) / det;
+        sl[i] = (-DCmm * el + DCm0 * er) / det;
+        sr[i] = (DC0m * el - DC00 * er) / det;
+    }
+    __syncthreads();
+    const int n = m - 2;
+    for (long e = tid; e < (long)n * n; e += kAuxThreads) {
+        const int i = 1 + (int)(e % n), j = 1 + (int)(e / n);   // i = y, j = z
+        const size_t idx = (size_t)i + (size_t)j * m;
+        out[idx] = in[idx] + DC[(size_t)j * m] * sb[i] + DC[(m - 1) + (size_t)j * m] * stt[i] +
+                   DC[(size_t)i * m] * sl[j] + DC[(m - 1) + (size_t)i * m] * sr[j];
+    }
+}
+
+hipError_t launch_boundary_adj(const BoundaryAdjArgs& a, hipStream_t st)
+{
+    const int planes = a.g.nf * a.g.nmodes;
+    if (planes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_boundary_adj, dim3(planes), dim3(kAuxThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kAuxThreads) void k_gauge_adj(GaugeAdjArgs a, GaugeList list)
+{
+    const int plane = list.plane[blockIdx.x];
+    const int fm = plane >> 1, part = plane & 1;
+    const FieldGroup& g = a.g;
+    const int f = fm / g.nmodes, mode = fm % g.nmodes, m = g.m, n = g.n, NP = a.NP, io = a.gauge;
+    const double* U = (const double*)(g.in[f] + (size_t)mode * m * m) + part;   // stride 2 doubles
+    double* G = a.G + (size_t)plane * NP * NP;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double s = 0.0;
+    for (int k = wave; k < n; k += kAuxWaves) {
+        const double pk = a.pcol[k];
+        for (int l = lane; l < n; l += 64) s += a.pcol[l] * U[2 * ((size_t)(l + 1) + (size_t)(k + 1) * m)] * pk;
+    }
+    __shared__ double red[kAuxWaves];
+    __shared__ double cc;
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double v = 0.0;
+        for (int w = 0; w < kAuxWaves; ++w) v += red[w];
+        cc = v / (a.pdiag * a.pdiag);
+    }
+    __syncthreads();
+    const double c = cc;
+    const double xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
+    for (int k = wave; k < n; k += kAuxWaves)
+        for (int l = lane; l < n; l += 64) {
+            if (l == io && k == io) continue;
+            G[l + (size_t)k * NP] -= c * a.prow[l] * a.prow[k] / (a.lam[l] + a.lam[k] - xlam);
+        }
+}
+
+hipError_t launch_gauge_adj(const GaugeAdjArgs& a, const GaugeList& list, hipStream_t st)
+{
+    if (list.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gauge_adj, dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
     return hipGetLastError();
 }
 

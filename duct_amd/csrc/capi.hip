@@ -39,6 +39,12 @@ struct DevBasis {
     double* opfrag = nullptr;// interior operator D2B, FP64 fragment layout (residual)
     double* qrow = nullptr;  // P^-1(iofzero, :), Neumann null projection
     double* pcol = nullptr;  // P(:, iofzero)
+    double* AfwdT = nullptr; // P^T,  fragment layout (adjoint passes 1, 2)
+    double* AbwdT = nullptr; // P^-T, fragment layout (adjoint passes 3, 4)
+    int NH = 0;              // folded (even/odd) size, 0 if the basis does not fold
+    double* ffwd = nullptr;  // folded P^-1, (2 NH) x NH fragment layout
+    double* fbwd = nullptr;  // folded P
+    double* flam = nullptr;  // eigenvalues per parity, 2 NH
     double pdiag = 0.0, wanted = 0.0;
     int gauge = -1;          // 0-based eigen-index of the null mode (Neumann)
 };
@@ -50,7 +56,8 @@ struct ProfRec {
 
 enum ProfKind { PK_C2P = 0, PK_P2P_DIV = 1, PK_GAUGE = 2, PK_P2P = 3, PK_P2C = 4, PK_BOUNDARY = 5, PK_BCCORR = 6,
                 PK_TRANSPOSE = 7, PK_INVDIAG = 8, PK_LP_C2P = 9, PK_LP_P2P_DIV = 10, PK_LP_P2P = 11,
-                PK_LP_P2C = 12, PK_RESID = 13 };
+                PK_LP_P2C = 12, PK_RESID = 13, PK_DERIV = 14, PK_ADJ = 15, PK_F_C2P = 16,
+                PK_F_P2P_DIV = 17, PK_F_P2P = 18, PK_F_P2C = 19 };
 
 struct Lib {
     bool init = false;
@@ -64,6 +71,9 @@ struct Lib {
     d2v* rfield = nullptr;                // residual fields of the mixed solve
     size_t rfield_bytes = 0;
     unsigned long long* stats = nullptr;  // residual maxima of the mixed solve
+    double* dcfrag[2] = {nullptr, nullptr}; // DC and DC^T, FP64 fragment layout (derivative products)
+    int dc_m = 0, dc_np = 0;
+    int fold = 0;                         // 1: Dirichlet groups use the folded passes
     size_t stats_bytes = 0;
     bool prof = false;
     std::vector<ProfRec> recs;
@@ -85,6 +95,7 @@ void free_basis(DevBasis& B)
     free_ptr(B.Afwd); free_ptr(B.Abwd); free_ptr(B.lam); free_ptr(B.DC); free_ptr(B.AM1);
     free_ptr(B.D2b); free_ptr(B.prow); free_ptr(B.lpf_fwd); free_ptr(B.lpf_bwd); free_ptr(B.lph_fwd);
     free_ptr(B.lph_bwd); free_ptr(B.opfrag); free_ptr(B.qrow); free_ptr(B.pcol);
+    free_ptr(B.AfwdT); free_ptr(B.AbwdT); free_ptr(B.ffwd); free_ptr(B.fbwd); free_ptr(B.flam);
     B = DevBasis{};
 }
 
@@ -189,11 +200,55 @@ int bound_index(char c)
 
 // The four projection passes, the gauge and the boundary for one group of
 // fields sharing a basis.  bmode is a BoundaryMode.
+// The four passes in the even/odd folded basis (Dirichlet), then the boundary.
+int run_group_fold(const FieldGroup& g, int bidx, int bmode)
+{
+    DevBasis& B = L.basis[bidx];
+    const int NH = B.NH;
+    const long cols = (long)g.nf * g.nmodes * NH;
+    const long planes = (long)g.nf * g.nmodes * 2;
+    int st = ensure_planar((size_t)planes * 4 * NH * NH * sizeof(double));
+    if (st) return st;
+    ProjFoldArgs a{};
+    a.g = g;
+    a.lam = B.flam;
+    a.ncols = cols;
+    {
+        Timed t(PK_F_C2P);
+        a.Afrag = B.ffwd; a.X = nullptr; a.Y = L.plan[0];
+        HIPCHK(launch_proj_fold(NH, PROJ_C2P, a, L.stream));
+    }
+    {
+        Timed t(PK_F_P2P_DIV);
+        a.X = L.plan[0]; a.Y = L.plan[1];
+        HIPCHK(launch_proj_fold(NH, PROJ_P2P_DIV, a, L.stream));
+    }
+    {
+        Timed t(PK_F_P2P);
+        a.Afrag = B.fbwd; a.X = L.plan[1]; a.Y = L.plan[0];
+        HIPCHK(launch_proj_fold(NH, PROJ_P2P, a, L.stream));
+    }
+    {
+        Timed t(PK_F_P2C);
+        a.X = L.plan[0]; a.Y = nullptr;
+        HIPCHK(launch_proj_fold(NH, PROJ_P2C, a, L.stream));
+    }
+    {
+        Timed t(PK_BOUNDARY);
+        BoundaryArgs ba{};
+        ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
+        ba.mode = bmode;
+        HIPCHK(launch_boundary(ba, L.stream));
+    }
+    return OK;
+}
+
 int run_group(const FieldGroup& g, int bidx, int bmode)
 {
     DevBasis& B = L.basis[bidx];
     if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
     if (g.m != B.m) return fail(ERR_ARG, "m=%d but the '%c' basis was set up for m=%d", g.m, bidx ? 'n' : 'd', B.m);
+    if (bidx == 0 && L.fold && B.NH > 0) return run_group_fold(g, bidx, bmode);
     const int NP = B.NP;
     const long cplx_cols = (long)g.nf * g.nmodes * NP;
     const long planes = (long)g.nf * g.nmodes * 2;
@@ -261,6 +316,92 @@ int run_group(const FieldGroup& g, int bidx, int bmode)
 }
 
 int check_common(int m, int nmodes);
+
+// Adjoint of run_group (the transposed solve): out = S^T in, boundary of out = 0.
+// For Neumann the boundary of `in` is first folded into its interior (E^T) in
+// the residual scratch, and the gauge adjoint corrects the singular planes.
+int run_group_adj(const FieldGroup& g0, int bidx)
+{
+    DevBasis& B = L.basis[bidx];
+    if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
+    if (g0.m != B.m) return fail(ERR_ARG, "m=%d but the basis was set up for m=%d", g0.m, B.m);
+    const int NP = B.NP, m = g0.m;
+    FieldGroup g = g0;
+    int st;
+    if (bidx == 1) {
+        const size_t field_bytes = (size_t)g.nmodes * m * m * sizeof(d2v);
+        if ((st = ensure_buf(L.rfield, L.rfield_bytes, field_bytes * g.nf))) return st;
+        BoundaryAdjArgs ba{};
+        ba.g = g0;
+        ba.DC = B.DC;
+        ba.AM1 = B.AM1;
+        for (int i = 0; i < g.nf; ++i) ba.g.out[i] = L.rfield + (size_t)i * g.nmodes * m * m;
+        {
+            Timed t(PK_ADJ);
+            HIPCHK(launch_boundary_adj(ba, L.stream));
+        }
+        for (int i = 0; i < g.nf; ++i) g.in[i] = ba.g.out[i];
+    }
+    const long cplx_cols = (long)g.nf * g.nmodes * NP;
+    const long planes = (long)g.nf * g.nmodes * 2;
+    if ((st = ensure_planar((size_t)planes * NP * NP * sizeof(double)))) return st;
+    ProjArgs a{};
+    a.g = g;
+    a.lam = B.lam;
+    a.gauge = bidx == 1 ? B.gauge : -1;
+    {
+        Timed t(PK_C2P);
+        a.Afrag = B.AfwdT; a.X = nullptr; a.Y = L.plan[0]; a.ncols = cplx_cols;
+        HIPCHK(launch_proj(NP, PROJ_C2P, a, L.stream));
+    }
+    {
+        Timed t(PK_P2P_DIV);
+        a.X = L.plan[0]; a.Y = L.plan[1]; a.ncols = planes * NP;
+        HIPCHK(launch_proj(NP, PROJ_P2P_DIV, a, L.stream));
+    }
+    if (bidx == 1) {
+        GaugeAdjArgs ga{};
+        ga.g = g; ga.G = L.plan[1]; ga.prow = B.prow; ga.pcol = B.pcol; ga.lam = B.lam;
+        ga.pdiag = B.pdiag; ga.gauge = B.gauge; ga.NP = NP;
+        GaugeList gl{};
+        for (int f = 0; f < g.nf; ++f)
+            for (int k = 0; k < g.nmodes; ++k) {
+                if (!(std::fabs(xlambda_of(g.alambda[f], g.alp, g.mode0 + k)) <= 1.0e-10)) continue;
+                for (int part = 0; part < 2; ++part) {
+                    if (part == 1 && g.mode0 + k == 0) continue;
+                    gl.plane[gl.count++] = 2 * (f * g.nmodes + k) + part;
+                    if (gl.count == kMaxGaugePlanes) {
+                        Timed t(PK_GAUGE);
+                        HIPCHK(launch_gauge_adj(ga, gl, L.stream));
+                        gl.count = 0;
+                    }
+                }
+            }
+        if (gl.count) {
+            Timed t(PK_GAUGE);
+            HIPCHK(launch_gauge_adj(ga, gl, L.stream));
+        }
+    }
+    {
+        Timed t(PK_P2P);
+        a.Afrag = B.AbwdT; a.X = L.plan[1]; a.Y = L.plan[0]; a.ncols = planes * NP;
+        HIPCHK(launch_proj(NP, PROJ_P2P, a, L.stream));
+    }
+    {
+        Timed t(PK_P2C);
+        a.X = L.plan[0]; a.Y = nullptr; a.ncols = cplx_cols;
+        HIPCHK(launch_proj(NP, PROJ_P2C, a, L.stream));
+    }
+    {
+        Timed t(PK_BOUNDARY);
+        BoundaryArgs ba{};
+        ba.g = g0; ba.DC = B.DC; ba.AM1 = B.AM1;
+        ba.mode = BND_OUT_ZERO;
+        HIPCHK(launch_boundary(ba, L.stream));
+    }
+    return OK;
+}
+
 
 // Low-precision pass chain for one field group: sol interior = (or +=) S(in).
 int run_group_lp(const FieldGroup& g, int bidx, int scheme, int accumulate, double wanted)
@@ -489,6 +630,9 @@ int duct_shutdown_(void)
     for (auto& p : L.hfield) free_ptr(p);
     free_ptr(L.rfield);
     free_ptr(L.stats);
+    free_ptr(L.dcfrag[0]);
+    free_ptr(L.dcfrag[1]);
+    L.dc_m = L.dc_np = 0;
     L.plan_bytes = L.hfield_bytes = L.rfield_bytes = L.stats_bytes = 0;
     for (auto& r : L.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto& e : L.pool) (void)hipEventDestroy(e);
@@ -573,6 +717,43 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
     };
     if ((st = up(B.Afwd, fwd.data(), fwd.size()))) return st;
     if ((st = up(B.Abwd, bwd.data(), bwd.size()))) return st;
+    {   // adjoint bases: P^T and P^-T
+        std::vector<double> T((size_t)n * n);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) T[i + (size_t)j * n] = P[j + (size_t)i * n];
+        build_frag(T.data(), n, n, NP, fwd.data());
+        if ((st = up(B.AfwdT, fwd.data(), fwd.size()))) return st;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) T[i + (size_t)j * n] = Pinv[j + (size_t)i * n];
+        build_frag(T.data(), n, n, NP, bwd.data());
+        if ((st = up(B.AbwdT, bwd.data(), bwd.size()))) return st;
+    }
+    {   // collocation derivative matrix DC and its transpose, padded to a multiple of 16
+        const int MP = (m + 15) / 16 * 16;
+        std::vector<double> f((size_t)MP * MP), T((size_t)m * m);
+        free_ptr(L.dcfrag[0]);
+        free_ptr(L.dcfrag[1]);
+        build_frag(DC, m, m, MP, f.data());
+        if ((st = up(L.dcfrag[0], f.data(), f.size()))) return st;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) T[i + (size_t)j * m] = DC[j + (size_t)i * m];
+        build_frag(T.data(), m, m, MP, f.data());
+        if ((st = up(L.dcfrag[1], f.data(), f.size()))) return st;
+        L.dc_m = m;
+        L.dc_np = MP;
+    }
+    if (bidx == 0) {   // even/odd folded bases (used when duct_set_fold_ enables them)
+        const int NH = fold_size(n);
+        if (NH > 0) {
+            std::vector<double> ff((size_t)2 * NH * NH), fb((size_t)2 * NH * NH), fl((size_t)2 * NH);
+            if (build_fold_bases(P, Pinv, lam, n, NH, ff.data(), fb.data(), fl.data()) == 0) {
+                if ((st = up(B.ffwd, ff.data(), ff.size()))) return st;
+                if ((st = up(B.fbwd, fb.data(), fb.size()))) return st;
+                if ((st = up(B.flam, fl.data(), fl.size()))) return st;
+                B.NH = NH;
+            }
+        }
+    }
     if ((st = up(B.lam, lp.data(), lp.size()))) return st;
     if ((st = up(B.DC, DC, (size_t)m * m))) return st;
     if ((st = up(B.AM1, am1.data(), 16))) return st;
@@ -766,6 +947,26 @@ int duct_solvediag_(double* U, const char* bound, const double* xlambda, size_t 
     return OK;
 }
 
+int duct_set_fold_(const int* mode)
+{
+    if (!mode) return fail(ERR_ARG, "null argument");
+    if (*mode != 0 && *mode != 1) return fail(ERR_ARG, "fold mode %d is neither 0 nor 1", *mode);
+    int st = ensure_init();
+    if (st) return st;
+    L.fold = *mode;
+    return OK;
+}
+
+int duct_fold_size_(const char* bound, int* NH, size_t bound_len)
+{
+    (void)bound_len;
+    if (!bound || !NH) return fail(ERR_ARG, "null argument");
+    const int b = bound_index(*bound);
+    if (b < 0) return fail(ERR_ARG, "bound '%c' is neither 'd' nor 'n'", *bound);
+    *NH = L.basis[b].ready ? L.basis[b].NH : 0;
+    return OK;
+}
+
 int duct_prof_enable_(const int* on)
 {
     if (!on) return fail(ERR_ARG, "null argument");
@@ -864,6 +1065,81 @@ int duct_helm3d_mixed_batch_dev_(const int* nfields, const duct_devptr_t* rhs, c
     if (st) return st;
     return solve_fields_mixed(*nfields, rhs, sol, bounds, alambda, *m, *nmodes, *ib, *alp, *scheme0, *scheme1,
                               *niter, stats);
+}
+
+int duct_helm3d_adjoint_batch_dev_(const int* nfields, const duct_devptr_t* solbar, const duct_devptr_t* rhsbar,
+                                   const char* bounds, const double* alambda, const int* m, const int* nmodes,
+                                   const int* ib, const double* alp, size_t bounds_len)
+{
+    if (!nfields || !solbar || !rhsbar || !bounds || !alambda || !m || !nmodes || !ib || !alp)
+        return fail(ERR_ARG, "null argument to duct_helm3d_adjoint_batch_dev_");
+    if (bounds_len && (size_t)*nfields > bounds_len)
+        return fail(ERR_ARG, "bounds has %zu characters for %d fields", bounds_len, *nfields);
+    const int nf = *nfields;
+    if (nf < 1 || nf > kMaxFields) return fail(ERR_ARG, "nfields=%d outside [1, %d]", nf, kMaxFields);
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    for (int bidx = 0; bidx < 2; ++bidx) {
+        FieldGroup g{};
+        g.nmodes = *nmodes; g.mode0 = *ib; g.m = *m; g.n = *m - 2; g.alp = *alp;
+        for (int f = 0; f < nf; ++f) {
+            const int bi = bound_index(bounds[f]);
+            if (bi < 0) return fail(ERR_ARG, "bound '%c' of field %d is neither 'd' nor 'n'", bounds[f], f);
+            if (bi != bidx) continue;
+            if (!solbar[f] || !rhsbar[f]) return fail(ERR_ARG, "null device pointer for field %d", f);
+            if (solbar[f] == rhsbar[f]) return fail(ERR_ARG, "adjoint input and output of field %d alias", f);
+            g.in[g.nf] = (const d2v*)solbar[f];
+            g.out[g.nf] = (d2v*)rhsbar[f];
+            g.alambda[g.nf] = alambda[f];
+            ++g.nf;
+        }
+        if (g.nf == 0) continue;
+        if ((st = run_group_adj(g, bidx))) return st;
+    }
+    return OK;
+}
+
+int duct_colloc_deriv_dev_(const duct_devptr_t* in, const duct_devptr_t* out, const int* m, const int* nmodes,
+                           const char* cdir, const int* trans, const double* scal, const double* beta,
+                           size_t cdir_len)
+{
+    if (!in || !out || !m || !nmodes || !cdir || !trans || !scal || !beta)
+        return fail(ERR_ARG, "null argument to duct_colloc_deriv_dev_");
+    (void)cdir_len;
+    if (*cdir != 'y' && *cdir != 'z') return fail(ERR_ARG, "cdir '%c' is neither 'y' nor 'z'", *cdir);
+    if (!*in || !*out) return fail(ERR_ARG, "null device pointer");
+    if (*in == *out) return fail(ERR_ARG, "in and out must not alias");
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    if (!L.dcfrag[0] || L.dc_m != *m)
+        return fail(ERR_NOT_SETUP, "no collocation matrix for m=%d (duct_setup_ builds it)", *m);
+    CollocArgs a{};
+    a.g.nf = 1; a.g.nmodes = *nmodes; a.g.m = *m; a.g.n = *m - 2;
+    a.g.in[0] = (const d2v*)*in;
+    a.g.out[0] = (d2v*)*out;
+    a.Dfrag = L.dcfrag[*trans ? 1 : 0];
+    a.NP = L.dc_np;
+    a.scal = *scal;
+    a.beta = *beta;
+    Timed t(PK_DERIV);
+    HIPCHK(launch_colloc(*cdir == 'y' ? CL_DY : CL_DZ, a, L.stream));
+    return OK;
+}
+
+int duct_derivx_dev_(const duct_devptr_t* in, const duct_devptr_t* out, const int* m, const int* nmodes,
+                     const int* ib, const double* alp, const double* scal, const double* beta)
+{
+    if (!in || !out || !m || !nmodes || !ib || !alp || !scal || !beta)
+        return fail(ERR_ARG, "null argument to duct_derivx_dev_");
+    if (!*in || !*out) return fail(ERR_ARG, "null device pointer");
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    Timed t(PK_DERIV);
+    HIPCHK(launch_derivx((const d2v*)*in, (d2v*)*out, *m, *nmodes, *ib, *alp, *scal, *beta, L.stream));
+    return OK;
 }
 
 }  // extern "C"

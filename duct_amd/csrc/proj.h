@@ -38,12 +38,35 @@ struct ProjArgs {
 
 hipError_t launch_proj(int NP, int mode, const ProjArgs& a, hipStream_t st);
 void build_frag(const double* A, int n, int lda, int NP, double* frag);
+// Fragment layout of a general R x K matrix (R, K multiples of 16 and 8).
+void build_frag_rect(const double* A, int R, int K, int lda, double* frag);
+
+// Even/odd (folded) projection passes: planes of four NH x NH quarters,
+// basis = [A_e; A_o] as a (2 NH) x NH fragment-layout matrix.
+struct ProjFoldArgs {
+    const double* Afrag;  // folded basis, fragment layout
+    const double* X;      // planar input (quarters)
+    double* Y;            // planar output (quarters)
+    const double* lam;    // [lam_e | lam_o], NH each, zero padded
+    long ncols;           // folded complex columns: nf * nmodes * NH
+    int accumulate;       // pass 4: out += U instead of out = U
+    FieldGroup g;
+};
+hipError_t launch_proj_fold(int NH, int mode, const ProjFoldArgs& a, hipStream_t st);
+// Padded folded size NH for interior n, or -1 when the folded kernels do not apply.
+int fold_size(int n);
+// Classify P's columns by parity and build the folded P^-1 (fwd) and P (bwd)
+// fragments, each (2 NH) x NH, and the eigenvalues per parity.  Returns -1 if
+// the eigenvectors do not split into ceil(n/2) even and floor(n/2) odd ones.
+int build_fold_bases(const double* P, const double* Pinv, const double* lam, int n, int NH,
+                     double* fwd, double* bwd, double* lamf);
 
 // Boundary treatment after pass 4 (one workgroup per complex plane).
 enum BoundaryMode : int {
     BND_DIR_ZERO = 0,  // helm3d.f90:48-60: rhs and sol boundary set to zero
     BND_DIR_KEEP = 1,  // solvediag.f90 'd': boundary of U left as given (copied in -> out)
     BND_NEUMANN = 2,   // solvediag.f90:251-322: edges from DC rows, corners from AM1
+    BND_OUT_ZERO = 3,  // adjoint solve: sol boundary set to zero, rhs untouched
 };
 
 struct BoundaryArgs {
@@ -87,6 +110,30 @@ struct NullProjArgs {
     int nmodes, m, n;
 };
 hipError_t launch_nullproj(const NullProjArgs& a, const GaugeList& list, hipStream_t st);
+
+// Adjoint of the Neumann boundary reconstruction (solvediag.f90:251-322):
+// out_interior = in_interior + E^T in_boundary, E mapping the interior to the
+// edges (DC rows) and corners (AM1).  One workgroup per complex plane.
+struct BoundaryAdjArgs {
+    FieldGroup g;       // g.in = adjoint field (full plane), g.out = interior result
+    const double* DC;   // m x m
+    const double* AM1;  // 4 x 4
+};
+hipError_t launch_boundary_adj(const BoundaryAdjArgs& a, hipStream_t st);
+
+// Adjoint of the k=0 gauge on planar pass-2 output G = (P^T Ubar P) ./ den:
+//   G(l,k) -= c P(a,l) P(a,k) / den(l,k) for (l,k) != (a,a),  c = P(:,a)^T Ubar P(:,a) / P(a,a)^2
+// (forward: G(a,a) = (wanted - sum_{(l,k) != (a,a)} P(a,l) P(a,k) G(l,k)) / P(a,a)^2).
+struct GaugeAdjArgs {
+    FieldGroup g;        // g.in = adjoint input field Ubar (interior read)
+    double* G;           // planar pass-2 output
+    const double* prow;  // P(a, :)
+    const double* pcol;  // P(:, a)
+    const double* lam;   // eigenvalues
+    double pdiag;
+    int gauge, NP;
+};
+hipError_t launch_gauge_adj(const GaugeAdjArgs& a, const GaugeList& list, hipStream_t st);
 
 // Dirichlet boundary-condition correction of the right-hand side
 // (solvediag.f90:37-45), applied in place to the interior of a complex field.

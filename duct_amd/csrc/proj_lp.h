@@ -30,15 +30,27 @@ size_t lp_frag_bytes(int NP, int scheme);
 void build_frag_lp(const double* A, int n, int lda, int NP, int scheme, void* out);
 
 // r = f - (D2B u + u D2B^T - xlambda u) on the interior, FP64.
-struct ResidArgs {
-    FieldGroup g;                 // g.in = f (right-hand sides), g.out = u (current solutions)
-    d2v* r[kMaxFields];           // residual fields (interior written)
-    const double* Dfrag;          // D2B (n x n) in the FP64 fragment layout of build_frag, NP padded
-    unsigned long long* stats;    // bit patterns of max|r|, max|f| (atomicMax) at stats[2*slot[f]], or null
-    int neumann;                  // singular planes (|xlambda| <= 1e-10) leave max|r| to k_nullproj
-    int slot[kMaxFields];         // stats slot of each field
-    int NP;
+// FP64 collocation-operator products (resid.hip).
+enum CollocMode : int {
+    CL_RESID = 0,   // r = f - (D2B_y u + u D2B_z^T - xlambda u), interior
+    CL_DY = 1,      // out = beta out + scal (D u) along y, full plane
+    CL_DZ = 2,      // out = beta out + scal (u D^T) along z, full plane
 };
+
+struct CollocArgs {
+    FieldGroup g;                 // RESID: g.in = f, g.out = u.  DY/DZ: g.in = operand, g.out = result
+    d2v* r[kMaxFields];           // RESID: residual fields (interior written)
+    const double* Dfrag;          // the matrix in the FP64 fragment layout of build_frag, NP padded
+    unsigned long long* stats;    // RESID: bit patterns of max|r|, max|f| at stats[2*slot[f]], or null
+    int neumann;                  // RESID: singular planes (|xlambda| <= 1e-10) leave max|r| to k_nullproj
+    int slot[kMaxFields];         // RESID: stats slot of each field
+    int NP;                       // padded matrix size (multiple of 16)
+    double scal, beta;            // DY/DZ
+};
+typedef CollocArgs ResidArgs;
+hipError_t launch_colloc(int mode, const CollocArgs& a, hipStream_t st);
 hipError_t launch_resid(const ResidArgs& a, hipStream_t st);
+hipError_t launch_derivx(const d2v* in, d2v* out, int m, int nmodes, int mode0, double alp, double scal, double beta,
+                         hipStream_t st);
 
 }  // namespace duct

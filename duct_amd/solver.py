@@ -147,6 +147,18 @@ def solvediag(U: np.ndarray, bound: str, xlambda: float) -> np.ndarray:
     return out
 
 
+def set_fold(on: bool) -> None:
+    """Dirichlet solves in the even/odd folded eigenbasis (duct_set_fold_)."""
+    _lib.call("duct_set_fold_", _lib.byref_int(1 if on else 0))
+
+
+def fold_size(bound: str) -> int:
+    """Folded size NH of the basis set up for bound (0: the basis does not fold)."""
+    v = ctypes.c_int(0)
+    _lib.call("duct_fold_size_", _bound_bytes(bound), ctypes.byref(v), ctypes.c_size_t(1))
+    return v.value
+
+
 class KernelTimer:
     """Per-kernel-class device time from hipEvents the library records on its stream."""
 

@@ -130,6 +130,34 @@ int duct_helm3d_mixed_batch_dev_(const int *nfields, const duct_devptr_t *rhs,
                                  const int *scheme1, const int *niter, double *stats,
                                  size_t bounds_len);
 
+/* Adjoint (transposed) solve of duct_helm3d_batch_dev_: rhsbar = S^T solbar for
+ * every field, S the linear map rhs -> sol of the forward solve including its
+ * boundary treatment (Dirichlet: rhs boundary ignored, sol boundary zero;
+ * Neumann: sol boundary from the edge/corner equations, k=0 gauge).  The
+ * reference has no adjoint; this is the backward of the solve for gradients
+ * of flow functionals (solvediag.f90 with P -> P^-T and P^-1 -> P^T).
+ * rhsbar's boundary is written as zero; solbar is not modified.            */
+int duct_helm3d_adjoint_batch_dev_(const int *nfields, const duct_devptr_t *solbar,
+                                   const duct_devptr_t *rhsbar, const char *bounds,
+                                   const double *alambda, const int *m, const int *nmodes,
+                                   const int *ib, const double *alp, size_t bounds_len);
+
+/* Collocation derivative products on complex(8) fields (m, m, nmodes), full
+ * planes: out = beta*out + scal * (D u) along y (cdir 'y') or (u D^T) along z
+ * ('z'), D = DC (trans = 0) or DC^T (trans = 1), DC the CGL derivative matrix
+ * uploaded by duct_setup_ (pseudomat.f90:90-107, cheb_collocation.F:19-75).
+ * d/dy, d/dz of the nodal interpolant: the physical-space equivalent of
+ * transyz + derivyz (chebyshev.f90:1-295, 458-540).  in and out must differ. */
+int duct_colloc_deriv_dev_(const duct_devptr_t *in, const duct_devptr_t *out, const int *m,
+                           const int *nmodes, const char *cdir, const int *trans,
+                           const double *scal, const double *beta, size_t cdir_len);
+
+/* Fourier x-derivative: out = beta*out + scal * (i*alp*k) * in for modes
+ * k = ib .. ib+nmodes-1 (derivx, src/fourier.F:117-188, ider = 1).          */
+int duct_derivx_dev_(const duct_devptr_t *in, const duct_devptr_t *out, const int *m,
+                     const int *nmodes, const int *ib, const double *alp, const double *scal,
+                     const double *beta);
+
 /* ---- bandwidth-bound helpers (reference GPU glue) ----------------------- */
 
 /* C = alpha * A^T for nbatch consecutive Nx x Ny FP64 matrices (column-major,
@@ -153,6 +181,22 @@ int duct_cyzx2xyz_dev_(const int *nx, const int *my, const int *nz, const duct_d
 int duct_mat_invdiag_dev_(const int *Nx, const int *Ny, const double *xlam, const duct_devptr_t *F,
                           const duct_devptr_t *ED2, const duct_devptr_t *C);
 
+/* ---- even/odd folded Dirichlet solve ------------------------------------ */
+
+/* 1: Dirichlet solves (duct_helm3d_dev_, _batch_dev_, duct_helm3d_,
+ * duct_int2d3d_, duct_solvediag_) run in the even/odd folded eigenbasis, half
+ * the FLOPs of the four products; 0 (default): the reference's unfolded
+ * products (solvediag.f90:51-80).  D2B commutes with the reflection of the
+ * CGL nodes, so PD's columns are even or odd; the folded solve uses their
+ * parity projections and agrees with the unfolded one to ~1e-13 relative at
+ * m = 257 (rounding of the reference's nearly-degenerate eigenvector pairs).
+ * The reference has no such switch.                                         */
+int duct_set_fold_(const int *mode);
+
+/* Folded size NH (padded ceil(n/2)) of the basis set up for bound, 0 when it
+ * does not fold (n < 65, or eigenvectors without a clean parity split).     */
+int duct_fold_size_(const char *bound, int *NH, size_t bound_len);
+
 /* ---- kernel timing (hipEvents on the library stream) -------------------- */
 
 /* Enable (1) / disable (0) per-launch event timing and clear the records. */
@@ -161,7 +205,8 @@ int duct_prof_enable_(const int *on);
  * device time in ms and the launch count.  Classes: 0 proj_c2p,
  * 1 proj_p2p_div, 2 gauge, 3 proj_p2p, 4 proj_p2c, 5 boundary, 6 bc_correct,
  * 7 transpose, 8 invdiag, 9 lp_c2p, 10 lp_p2p_div, 11 lp_p2p, 12 lp_p2c,
- * 13 resid. */
+ * 13 resid, 14 deriv, 15 adjoint (Neumann boundary adjoint), 16 fold_c2p,
+ * 17 fold_p2p_div, 18 fold_p2p, 19 fold_p2c. */
 int duct_prof_read_(const int *nclass, double *ms, int *count);
 
 #ifdef __cplusplus
