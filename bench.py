@@ -5,20 +5,26 @@ Workload (N=1): 257 x 257 CGL points x 256 complex Fourier modes x 4 fields
 k=0 gauge), seeded synthetic right-hand sides, inputs resident in HBM.
 One step = one batched solve of all four fields.
 
-Two paths are timed on the same inputs:
-  fp64   the plain FP64 form: four FP64-MFMA eigenbasis passes per boundary group
-  mixed  the refined form: low-precision passes (FP32 or split-bf16 MFMA),
-         FP64 residual, correction(s) accumulated into the FP64 solution
-The reported `value` is the faster path whose result is within 1e-10 (relative,
-max norm, every field) of the oracle; both paths' throughput, errors and the
-refinement count are in the line.
+Paths timed on the same inputs:
+  fp64         the plain FP64 form: four FP64-MFMA eigenbasis passes per boundary
+               group, the reference's unfolded products (agrees with the oracle to ~1e-15)
+  fp64_fold_d  the Dirichlet fields in the even/odd folded eigenbasis (half the FLOPs;
+               ~1e-13 of the oracle), the Neumann field unfolded
+  fp64_fold    every field folded, the Neumann k=0 gauge included (~5e-11 of the
+               oracle on the Poisson planes, their own conditioning)
+  mixed        the refined form: low-precision passes (FP32 or split-bf16 MFMA),
+               FP64 residual, correction(s) accumulated into the FP64 solution
+The reported `value` is the fastest path whose result is within 1e-10 (relative,
+max norm, every field) of the oracle; every path's throughput, error and kernel
+times are in the line.
 
 Metric: mode-solves/s, a mode-solve being one (field, k_x) complex 2-D solve.
 Roofline: the chosen path's dominant kernel class, algorithmic FLOP per launch
-(2 n^3 per real plane and product for a projection pass, 4 n^3 for the dense
-FP64 residual; n = m-2, padding and the unsolved imaginary plane at k=0 not
-counted) over its mean launch time from hipEvents recorded on the library's
-stream during the timed steps.
+(2 n^3 per real plane and product for an unfolded projection pass,
+2 (h_e^2 + h_o^2) n for a folded one with h_e = ceil(n/2), h_o = floor(n/2),
+4 n^3 for the dense FP64 residual; n = m-2, padding and the unsolved imaginary
+plane at k=0 not counted) over its mean launch time from hipEvents recorded on
+the library's stream during the timed steps.
 cpu_baseline: the oracle (C restatement of solvediag/int2d3d/helm3d, one BLAS
 dgemm per product) on the host cores over the whole workload; it also serves
 as the checker of both GPU results.
@@ -49,7 +55,7 @@ def parse():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--mixed", default="bf16x6,bf16x6,1",
                     help="scheme0,scheme1,niter of the refined solve")
-    ap.add_argument("--path", choices=["auto", "fp64", "mixed"], default="auto")
+    ap.add_argument("--path", choices=["auto", "fp64", "fp64_fold_d", "fp64_fold", "mixed"], default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -110,12 +116,23 @@ def main():
     alams = [workloads.alambda_of(cfg, f) for f in range(F)]
     bounds = workloads.BOUNDS[:F]
     rhs = [torch.from_numpy(x).cuda() for x in fields]
-    sol = {"fp64": [torch.zeros_like(r) for r in rhs], "mixed": [torch.zeros_like(r) for r in rhs]}
+    paths = ("fp64", "fp64_fold_d", "fp64_fold", "mixed")
+    sol = {p: [torch.zeros_like(r) for r in rhs] for p in paths}
+
+    def fp64_step(path, fold):
+        def run():
+            duct_amd.set_fold(fold)
+            duct_amd.helm3d_batch(rhs, sol[path], bounds, alams, alp=cfg.alp, ib=0)
+        return run
+
     steps = {
-        "fp64": lambda: duct_amd.helm3d_batch(rhs, sol["fp64"], bounds, alams, alp=cfg.alp, ib=0),
+        "fp64": fp64_step("fp64", 0),
+        "fp64_fold_d": fp64_step("fp64_fold_d", 1),
+        "fp64_fold": fp64_step("fp64_fold", 2),
         "mixed": lambda: duct_amd.helm3d_mixed_batch(rhs, sol["mixed"], bounds, alams, alp=cfg.alp, ib=0,
                                                      scheme0=s0, scheme1=s1, niter=niter),
     }
+    he, ho = (n + 1) // 2, n // 2
     planes = F * (2 * M - 1)                      # Im at k=0 not solved
     nprod = {"f32": 1, "bf16x1": 1, "bf16x3": 3, "bf16x6": 6}
 
@@ -124,6 +141,8 @@ def main():
         g = 2.0 * n ** 3 * planes                  # one projection pass, one product
         if kind.startswith("proj_"):
             return g
+        if kind.startswith("fold_"):               # the folded pass: 2 (h_e^2 + h_o^2) n per plane
+            return 2.0 * (he * he + ho * ho) * n * planes
         if kind.startswith("lp_"):
             return g * (nprod[s0] + niter * nprod[s1])
         if kind == "resid":
@@ -131,7 +150,7 @@ def main():
         return 0.0
 
     res = {}
-    for path in ("fp64", "mixed"):
+    for path in paths:
         for _ in range(args.warmup):
             steps[path]()
         torch.cuda.synchronize()
@@ -148,6 +167,7 @@ def main():
             classes[k] = {"launches_per_step": kt.count[k] / args.steps, "avg_us": kt.ms[k] / kt.count[k] * 1e3,
                           "ms_per_step": kt.ms[k] / args.steps}
         res[path] = {"ms_per_step": dt * 1e3, "mode_solves_per_s": F * M / dt, "kernels": classes}
+    duct_amd.set_fold(0)
 
     cpu, refs = None, None
     if not args.no_cpu_baseline:
@@ -167,13 +187,14 @@ def main():
     ks = res[chosen]["kernels"]
     groups = {}
     for k, v in ks.items():
-        key = "proj" if k.startswith("proj_") else "lp" if k.startswith("lp_") else k
+        key = "proj" if k.startswith("proj_") else "lp" if k.startswith("lp_") else \
+            "fold" if k.startswith("fold_") else k
         groups.setdefault(key, []).append(k)
     dom = max(groups, key=lambda g: sum(ks[k]["ms_per_step"] for k in groups[g]))
     dom_ms = sum(ks[k]["ms_per_step"] for k in groups[dom])
     dom_launches = sum(ks[k]["launches_per_step"] for k in groups[dom])
     dom_flop = sum(flop_per_step(k) for k in groups[dom])
-    unit_peak = {"proj": "f64", "resid": "f64", "lp": "f32" if s0 == "f32" else "bf16"}.get(dom, "f64")
+    unit_peak = {"proj": "f64", "fold": "f64", "resid": "f64", "lp": "f32" if s0 == "f32" else "bf16"}.get(dom, "f64")
     avg_launch_s = dom_ms / 1e3 / dom_launches if dom_launches else float("nan")
     flop_per_launch = dom_flop / dom_launches if dom_launches else 0.0
     achieved = flop_per_launch / avg_launch_s if dom_launches else 0.0
@@ -190,7 +211,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if chosen == "fp64" else "%s+%s refined, f64 residual" % (s0, s1),
+        "dtype": "f64" if chosen.startswith("fp64") else "%s+%s refined, f64 residual" % (s0, s1),
         "data": "synthetic (seeded Chebyshev-series right-hand sides, inputs resident in HBM)",
         "config": {"workload": "cross-plane solve stage: %d^2 CGL x %d modes x %d fields (u,v,w Dirichlet "
                                "alambda=1000; p Neumann alambda=0)" % (m, M, F),
@@ -199,6 +220,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[unit_peak] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[unit_peak], "traffic": None,
                      "kernel": {"proj": "k_proj (FP64 MFMA projection pass)",
+                                "fold": "k_projf (FP64 MFMA projection pass, even/odd folded basis)",
                                 "lp": "k_proj_lp (%s MFMA projection pass)" % unit_peak,
                                 "resid": "k_resid (FP64 MFMA residual)"}.get(dom, dom),
                      "flop_per_launch": flop_per_launch, "avg_launch_us": avg_launch_s * 1e6},

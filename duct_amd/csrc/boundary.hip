@@ -39,15 +39,26 @@ __global__ __launch_bounds__(kAuxThreads) void k_gauge(GaugeArgs a, GaugeList li
     const float* Gf = (const float*)a.G + base;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double part = 0.0;
-    for (int k = wave; k < n; k += kAuxWaves) {
-        const double pk = a.prow[k];
-        const size_t col = (size_t)k * NP;
-        if (a.f32) {
+    if (a.lam) {   // undivided G: divide each term as pass 2 would have (the (a,a) entry counts 0)
+        const int fm = plane >> 1;
+        const double xlam = xlambda_of(a.g.alambda[fm / a.g.nmodes], a.g.alp, a.g.mode0 + fm % a.g.nmodes);
+        for (int k = wave; k < n; k += kAuxWaves) {
+            const double pk = a.prow[k];
+            const size_t col = (size_t)k * NP;
+            for (int l = lane; l < n; l += 64)
+                if (!(l == io && k == io)) part += a.prow[l] * (Gd[col + l] / (a.lam[l] + a.lam[k] - xlam)) * pk;
+        }
+    } else {
+        for (int k = wave; k < n; k += kAuxWaves) {
+            const double pk = a.prow[k];
+            const size_t col = (size_t)k * NP;
+            if (a.f32) {
 #pragma unroll 4
-            for (int l = lane; l < n; l += 64) part += a.prow[l] * (double)Gf[col + l] * pk;
-        } else {
+                for (int l = lane; l < n; l += 64) part += a.prow[l] * (double)Gf[col + l] * pk;
+            } else {
 #pragma unroll 4
-            for (int l = lane; l < n; l += 64) part += a.prow[l] * Gd[col + l] * pk;
+                for (int l = lane; l < n; l += 64) part += a.prow[l] * Gd[col + l] * pk;
+            }
         }
     }
     __shared__ double red[kAuxWaves];
@@ -74,6 +85,53 @@ hipError_t launch_gauge(const GaugeArgs& a, const GaugeList& list, hipStream_t s
 {
     if (list.count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gauge, dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
+    return hipGetLastError();
+}
+
+// Gauge on the folded, undivided pass-2 output (quarters (r, c), element
+// (l, k) of a quarter at k NH + l, divided here by lam_r(l) + lam_c(k) - xlambda).  U(a,a) of the unfolded solution is the sum over the
+// quarters of s_r s_c (P_r G_rc P_c^T)(a', a'), a' the folded index of a and
+// s_o = -1 when a lies in the mirrored half; only the ee quarter holds the
+// null mode, so its (io, io) entry is set as in the unfolded gauge.
+__global__ __launch_bounds__(kAuxThreads) void k_gauge_fold(GaugeFoldArgs a, GaugeList list)
+{
+    const int plane = list.plane[blockIdx.x];
+    const int NH = a.NH, n = a.g.n;
+    const int hh[2] = {(n + 1) / 2, n / 2};
+    double* Gp = a.G + (size_t)plane * 4 * NH * NH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int fm = plane >> 1;
+    const double xlam = xlambda_of(a.g.alambda[fm / a.g.nmodes], a.g.alp, a.g.mode0 + fm % a.g.nmodes);
+    double part = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        const int r = q >> 1, c = q & 1;
+        const double* G = Gp + (size_t)q * NH * NH;
+        const double* pr = a.prow + r * NH;
+        const double* pc = a.prow + c * NH;
+        const double* lr = a.lam + r * NH;
+        const double* lc = a.lam + c * NH;
+        for (int k = wave; k < hh[c]; k += kAuxWaves) {
+            const double pk = pc[k];
+            for (int l = lane; l < hh[r]; l += 64)
+                if (!(q == 0 && l == a.io && k == a.io))
+                    part += pr[l] * (G[(size_t)k * NH + l] / (lr[l] + lc[k] - xlam)) * pk;
+        }
+    }
+    __shared__ double red[kAuxWaves];
+    part = wave_sum(part);
+    if (lane == 0) red[wave] = part;
+    __syncthreads();
+    if (tid == 0) {
+        double valore = 0.0;
+        for (int w = 0; w < kAuxWaves; ++w) valore += red[w];
+        Gp[(size_t)a.io * NH + a.io] = (a.wanted - valore) / (a.pdiag * a.pdiag);
+    }
+}
+
+hipError_t launch_gauge_fold(const GaugeFoldArgs& a, const GaugeList& list, hipStream_t st)
+{
+    if (list.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gauge_fold, dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
     return hipGetLastError();
 }
 

@@ -45,6 +45,9 @@ struct DevBasis {
     double* ffwd = nullptr;  // folded P^-1, (2 NH) x NH fragment layout
     double* fbwd = nullptr;  // folded P
     double* flam = nullptr;  // eigenvalues per parity, 2 NH
+    double* fprow = nullptr; // Neumann: signed folded rows of P at the gauge node, 2 NH
+    double fpdiag = 0.0;     // Neumann: P_e(a', io_e)
+    int fio = -1;            // Neumann: index of the null mode among the even eigenvectors
     double pdiag = 0.0, wanted = 0.0;
     int gauge = -1;          // 0-based eigen-index of the null mode (Neumann)
 };
@@ -73,7 +76,7 @@ struct Lib {
     unsigned long long* stats = nullptr;  // residual maxima of the mixed solve
     double* dcfrag[2] = {nullptr, nullptr}; // DC and DC^T, FP64 fragment layout (derivative products)
     int dc_m = 0, dc_np = 0;
-    int fold = 0;                         // 1: Dirichlet groups use the folded passes
+    int fold = 0;                         // 1: Dirichlet groups use the folded passes, 2: Neumann too
     size_t stats_bytes = 0;
     bool prof = false;
     std::vector<ProfRec> recs;
@@ -96,6 +99,7 @@ void free_basis(DevBasis& B)
     free_ptr(B.D2b); free_ptr(B.prow); free_ptr(B.lpf_fwd); free_ptr(B.lpf_bwd); free_ptr(B.lph_fwd);
     free_ptr(B.lph_bwd); free_ptr(B.opfrag); free_ptr(B.qrow); free_ptr(B.pcol);
     free_ptr(B.AfwdT); free_ptr(B.AbwdT); free_ptr(B.ffwd); free_ptr(B.fbwd); free_ptr(B.flam);
+    free_ptr(B.fprow);
     B = DevBasis{};
 }
 
@@ -213,20 +217,44 @@ int run_group_fold(const FieldGroup& g, int bidx, int bmode)
     a.g = g;
     a.lam = B.flam;
     a.ncols = cols;
+    a.gauge = bidx == 1 ? B.fio : -1;
     {
         Timed t(PK_F_C2P);
         a.Afrag = B.ffwd; a.X = nullptr; a.Y = L.plan[0];
         HIPCHK(launch_proj_fold(NH, PROJ_C2P, a, L.stream));
     }
-    {
+    {   // pass 2 undivided: the divide is applied by pass 3 as it loads G
         Timed t(PK_F_P2P_DIV);
         a.X = L.plan[0]; a.Y = L.plan[1];
-        HIPCHK(launch_proj_fold(NH, PROJ_P2P_DIV, a, L.stream));
+        HIPCHK(launch_proj_fold(NH, PROJ_P2P, a, L.stream));
+    }
+    if (bidx == 1) {  // singular planes: |xlambda| <= 1e-10 (solvediag.f90:151)
+        GaugeFoldArgs ga{};
+        ga.G = L.plan[1]; ga.prow = B.fprow; ga.pdiag = B.fpdiag; ga.wanted = B.wanted; ga.lam = B.flam;
+        ga.io = B.fio; ga.NH = NH; ga.g = g;
+        GaugeList gl{};
+        for (int f = 0; f < g.nf; ++f)
+            for (int k = 0; k < g.nmodes; ++k) {
+                if (!(std::fabs(xlambda_of(g.alambda[f], g.alp, g.mode0 + k)) <= 1.0e-10)) continue;
+                for (int part = 0; part < 2; ++part) {
+                    if (part == 1 && g.mode0 + k == 0) continue;
+                    gl.plane[gl.count++] = 2 * (f * g.nmodes + k) + part;
+                    if (gl.count == kMaxGaugePlanes) {
+                        Timed t(PK_GAUGE);
+                        HIPCHK(launch_gauge_fold(ga, gl, L.stream));
+                        gl.count = 0;
+                    }
+                }
+            }
+        if (gl.count) {
+            Timed t(PK_GAUGE);
+            HIPCHK(launch_gauge_fold(ga, gl, L.stream));
+        }
     }
     {
         Timed t(PK_F_P2P);
         a.Afrag = B.fbwd; a.X = L.plan[1]; a.Y = L.plan[0];
-        HIPCHK(launch_proj_fold(NH, PROJ_P2P, a, L.stream));
+        HIPCHK(launch_proj_fold(NH, PROJ_P2P_LDIV, a, L.stream));
     }
     {
         Timed t(PK_F_P2C);
@@ -237,7 +265,7 @@ int run_group_fold(const FieldGroup& g, int bidx, int bmode)
         Timed t(PK_BOUNDARY);
         BoundaryArgs ba{};
         ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
-        ba.mode = bmode;
+        ba.mode = bidx == 1 ? BND_NEUMANN : bmode;
         HIPCHK(launch_boundary(ba, L.stream));
     }
     return OK;
@@ -248,7 +276,7 @@ int run_group(const FieldGroup& g, int bidx, int bmode)
     DevBasis& B = L.basis[bidx];
     if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
     if (g.m != B.m) return fail(ERR_ARG, "m=%d but the '%c' basis was set up for m=%d", g.m, bidx ? 'n' : 'd', B.m);
-    if (bidx == 0 && L.fold && B.NH > 0) return run_group_fold(g, bidx, bmode);
+    if (B.NH > 0 && L.fold > bidx) return run_group_fold(g, bidx, bmode);
     const int NP = B.NP;
     const long cplx_cols = (long)g.nf * g.nmodes * NP;
     const long planes = (long)g.nf * g.nmodes * 2;
@@ -265,14 +293,15 @@ int run_group(const FieldGroup& g, int bidx, int bmode)
         a.Afrag = B.Afwd; a.X = nullptr; a.Y = L.plan[0]; a.ncols = cplx_cols;
         HIPCHK(launch_proj(NP, PROJ_C2P, a, L.stream));
     }
-    {   // pass 2: G = (P^-1 T)^T ./ (lam_i + lam_j - xlambda)
+    {   // pass 2: G = (P^-1 T)^T, undivided; pass 3 applies ./ (lam_i + lam_j - xlambda) as it loads G
         Timed t(PK_P2P_DIV);
         a.X = L.plan[0]; a.Y = L.plan[1]; a.ncols = planes * NP;
-        HIPCHK(launch_proj(NP, PROJ_P2P_DIV, a, L.stream));
+        HIPCHK(launch_proj(NP, PROJ_P2P, a, L.stream));
     }
     if (bidx == 1) {  // Neumann gauge planes: |xlambda| <= 1e-10 (solvediag.f90:151)
         GaugeArgs ga{};
         ga.G = L.plan[1]; ga.f32 = 0; ga.prow = B.prow; ga.pdiag = B.pdiag; ga.wanted = B.wanted;
+        ga.lam = B.lam;
         ga.gauge = B.gauge; ga.NP = NP; ga.g = g;
         GaugeList gl{};
         for (int f = 0; f < g.nf; ++f)
@@ -295,10 +324,10 @@ int run_group(const FieldGroup& g, int bidx, int bmode)
             HIPCHK(launch_gauge(ga, gl, L.stream));
         }
     }
-    {   // pass 3: T = (P G)^T
+    {   // pass 3: T = (P (G ./ den))^T
         Timed t(PK_P2P);
         a.Afrag = B.Abwd; a.X = L.plan[1]; a.Y = L.plan[0]; a.ncols = planes * NP;
-        HIPCHK(launch_proj(NP, PROJ_P2P, a, L.stream));
+        HIPCHK(launch_proj(NP, PROJ_P2P_LDIV, a, L.stream));
     }
     {   // pass 4: U = (P T)^T = P G P^T into the field interior
         Timed t(PK_P2C);
@@ -742,14 +771,33 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
         L.dc_m = m;
         L.dc_np = MP;
     }
-    if (bidx == 0) {   // even/odd folded bases (used when duct_set_fold_ enables them)
+    {   // even/odd folded bases (used when duct_set_fold_ enables them)
         const int NH = fold_size(n);
         if (NH > 0) {
             std::vector<double> ff((size_t)2 * NH * NH), fb((size_t)2 * NH * NH), fl((size_t)2 * NH);
-            if (build_fold_bases(P, Pinv, lam, n, NH, ff.data(), fb.data(), fl.data()) == 0) {
+            std::vector<int> pos(n);
+            bool ok = build_fold_bases(P, Pinv, lam, n, NH, ff.data(), fb.data(), fl.data(), pos.data()) == 0;
+            std::vector<double> fprow((size_t)2 * NH, 0.0);
+            if (ok && bidx == 1) {   // the null mode must be even; the gauge node a = iofzero-1
+                const int io = *iofzero - 1, he = (n + 1) / 2;
+                ok = pos[io] >= 0;
+                if (ok) {
+                    const int a0 = io < he ? io : n - 1 - io, a1 = n - 1 - a0;
+                    const double so = io < he ? 1.0 : -1.0;
+                    for (int k = 0; k < n; ++k) {
+                        const double pa = P[a0 + (size_t)k * n], pb = P[a1 + (size_t)k * n];
+                        if (pos[k] >= 0) fprow[pos[k]] = 0.5 * (pa + pb);
+                        else fprow[NH + (-1 - pos[k])] = so * 0.5 * (pa - pb);
+                    }
+                    B.fio = pos[io];
+                    B.fpdiag = fprow[pos[io]];
+                }
+            }
+            if (ok) {
                 if ((st = up(B.ffwd, ff.data(), ff.size()))) return st;
                 if ((st = up(B.fbwd, fb.data(), fb.size()))) return st;
                 if ((st = up(B.flam, fl.data(), fl.size()))) return st;
+                if (bidx == 1 && (st = up(B.fprow, fprow.data(), fprow.size()))) return st;
                 B.NH = NH;
             }
         }
@@ -950,7 +998,7 @@ int duct_solvediag_(double* U, const char* bound, const double* xlambda, size_t 
 int duct_set_fold_(const int* mode)
 {
     if (!mode) return fail(ERR_ARG, "null argument");
-    if (*mode != 0 && *mode != 1) return fail(ERR_ARG, "fold mode %d is neither 0 nor 1", *mode);
+    if (*mode < 0 || *mode > 2) return fail(ERR_ARG, "fold mode %d is not 0, 1 or 2", *mode);
     int st = ensure_init();
     if (st) return st;
     L.fold = *mode;

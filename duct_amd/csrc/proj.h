@@ -9,6 +9,8 @@ enum ProjMode : int {
     PROJ_P2P = 1,      // planar -> planar                  (pass 3)
     PROJ_P2P_DIV = 2,  // planar -> planar with the eigenvalue divide (pass 2)
     PROJ_P2C = 3,      // planar -> complex field interior  (pass 4)
+    PROJ_P2P_LDIV = 4, // planar -> planar, the eigenvalue divide applied to the
+                       // input as it is loaded (pass 3 after an undivided pass 2)
 };
 
 // Workgroup geometry of the projection kernels for a padded interior NP.
@@ -50,16 +52,21 @@ struct ProjFoldArgs {
     const double* lam;    // [lam_e | lam_o], NH each, zero padded
     long ncols;           // folded complex columns: nf * nmodes * NH
     int accumulate;       // pass 4: out += U instead of out = U
+    int gauge;            // pass 2: index of the Neumann null mode among the even
+                          // eigenvectors (its ee entry is left 0 on singular planes), -1 if none
     FieldGroup g;
 };
 hipError_t launch_proj_fold(int NH, int mode, const ProjFoldArgs& a, hipStream_t st);
+
 // Padded folded size NH for interior n, or -1 when the folded kernels do not apply.
 int fold_size(int n);
 // Classify P's columns by parity and build the folded P^-1 (fwd) and P (bwd)
 // fragments, each (2 NH) x NH, and the eigenvalues per parity.  Returns -1 if
 // the eigenvectors do not split into ceil(n/2) even and floor(n/2) odd ones.
+// pos (may be null) receives, for every eigenvector k, its index within its
+// parity (even: 0..h_e-1) or -1-index for odd ones.
 int build_fold_bases(const double* P, const double* Pinv, const double* lam, int n, int NH,
-                     double* fwd, double* bwd, double* lamf);
+                     double* fwd, double* bwd, double* lamf, int* pos);
 
 // Boundary treatment after pass 4 (one workgroup per complex plane).
 enum BoundaryMode : int {
@@ -86,6 +93,8 @@ struct GaugeArgs {
     double wanted;
     int relative;       // 1: target wanted - u(iofzero+1, iofzero+1) of the field in g.out
                         //    (a refinement correction pins the corrected sum, not itself)
+    const double* lam;  // non-null: G is undivided (pass 2 without the divide); each
+                        //    term is divided by lam_l + lam_k - xlambda as it is summed
     int gauge;          // 0-based iofzero-1
     int NP;
     FieldGroup g;
@@ -96,6 +105,20 @@ struct GaugeList {
     int plane[kMaxGaugePlanes];  // planar plane indices (2*(f*nmodes+mode)+part)
 };
 hipError_t launch_gauge(const GaugeArgs& a, const GaugeList& list, hipStream_t st);
+
+// Neumann k=0 gauge on folded pass-2 output: valore = sum over the four
+// quarters of s_r s_c prow_r^T G_rc prow_c, then G_ee(io, io) = (wanted - valore) / pdiag^2.
+struct GaugeFoldArgs {
+    double* G;            // folded planar pass-2 output
+    const double* prow;   // [s_e P_e(a', :) | s_o P_o(a', :)], NH each (sign of the mirror folded in)
+    double pdiag;         // P_e(a', io)
+    double wanted;
+    const double* lam;    // [lam_e | lam_o]: G is undivided, terms divided as they are summed
+    int io;               // index of the null mode among the even eigenvectors
+    int NH;
+    FieldGroup g;
+};
+hipError_t launch_gauge_fold(const GaugeFoldArgs& a, const GaugeList& list, hipStream_t st);
 
 // Null-space projection of the residual on the singular (Neumann, xlambda = 0)
 // planes of a refinement step: r -= (q^T r q) p p^T with q the null row of P^-1

@@ -10,12 +10,15 @@
 //
 // Storing the product transposed is what makes the chain close:
 //   pass 1 (C2P)      Y1 = (P^-1 F)^T                 complex field  -> planar
-//   pass 2 (P2P_DIV)  Y2 = (P^-1 Y1) ^T ./ den = G    planar -> planar, fused divide
-//   pass 3 (P2P)      Y3 = (P G)^T                    planar -> planar
+//   pass 2 (P2P)      Y2 = (P^-1 Y1)^T                planar -> planar
+//   pass 3 (P2P_LDIV) Y3 = (P (Y2 ./ den))^T          planar -> planar, divide on load
 //   pass 4 (P2C)      Y4 = (P Y3)^T = P G P^T = U     planar -> complex field
 // The complex<->real split of int2d3d.f90 (com2re/re2com) is fused into the
 // loads of pass 1 and the stores of pass 4; the divide of :61-66 / :151-159 is
-// the epilogue of pass 2.
+// applied to pass 3's input as it is loaded (the same rounding as dividing in
+// pass 2's epilogue, where it would stall every wave at once).  P2P_DIV, the
+// divide in pass 2's epilogue, serves the adjoint and the mixed solve, whose
+// gauge terms act on the divided G.
 //
 // Work decomposition.  A workgroup of 4 waves owns MT rows of A (all NP of them
 // for NP <= 256) and NT columns of the concatenated planes.  A (<= 2 MB) is
@@ -49,6 +52,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
     constexpr bool CIN = MODE == PROJ_C2P;
     constexpr bool COUT = MODE == PROJ_P2C;
     constexpr bool CPLX = CIN || COUT;
+    constexpr bool LDIV = MODE == PROJ_P2P_LDIV;
 
     __shared__ double Bs[2][NT * LDB];
 
@@ -74,13 +78,25 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
     int v_dst = 0, v_dst_im = 0, kk0 = 0;
     bool src_ok = false, zero_im = false;
 
-    if constexpr (MODE == PROJ_P2P || MODE == PROJ_P2P_DIV) {
+    // LDIV: column j of plane `plane` is divided by lam_j + lam_k - xlambda on load
+    int ld_j = 0;
+    double ld_xlam = 0.0;
+    bool ld_gauge = false;
+    if constexpr (MODE == PROJ_P2P || MODE == PROJ_P2P_DIV || LDIV) {
         const int v = tid / TPC;
         kk0 = (tid % TPC) * DPT;
         const long c = tile * NT + v;
         v_dst = v;
         src_ok = c < a.ncols;
         if (src_ok) srcp = a.X + (size_t)c * NP;
+        if constexpr (LDIV) {
+            const long plane = c / NP;
+            ld_j = (int)(c % NP);
+            const long fm = plane >> 1;
+            const int f = (int)(fm / g.nmodes), mode = (int)(fm % g.nmodes);
+            if (src_ok) ld_xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
+            ld_gauge = a.gauge >= 0 && fabs(ld_xlam) <= 1.0e-10 && ld_j == a.gauge;
+        }
     } else if constexpr (COUT) {
         const int v = tid / TPC;
         kk0 = (tid % TPC) * DPT;
@@ -130,6 +146,14 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
                 if (src_ok) v = *(const d2v*)(srcp + k0 + e);
                 breg[e] = v[0];
                 breg[e + 1] = v[1];
+            }
+            if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
+#pragma unroll
+                for (int e = 0; e < DPT; ++e) {
+                    const int k = k0 + e;
+                    if (ld_j < n && k < n && !(ld_gauge && k == a.gauge))
+                        breg[e] = breg[e] / (a.lam[ld_j] + a.lam[k] - ld_xlam);
+                }
             }
         }
     };
@@ -285,6 +309,7 @@ static hipError_t launch_np(int mode, const ProjArgs& a, hipStream_t st)
         case PROJ_P2P: hipLaunchKernelGGL((k_proj<NP, PROJ_P2P>), grid, block, 0, st, a); break;
         case PROJ_P2P_DIV: hipLaunchKernelGGL((k_proj<NP, PROJ_P2P_DIV>), grid, block, 0, st, a); break;
         case PROJ_P2C: hipLaunchKernelGGL((k_proj<NP, PROJ_P2C>), grid, block, 0, st, a); break;
+        case PROJ_P2P_LDIV: hipLaunchKernelGGL((k_proj<NP, PROJ_P2P_LDIV>), grid, block, 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

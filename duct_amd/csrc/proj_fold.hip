@@ -49,6 +49,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
     constexpr int NK = NH / S::KT;            // LDS stages over the folded contraction
     constexpr bool CIN = MODE == PROJ_C2P;
     constexpr bool COUT = MODE == PROJ_P2C;
+    constexpr bool LDIV = MODE == PROJ_P2P_LDIV;
     static_assert(TH >= 1 && NH % 64 == 0, "folded kernel needs NH >= 64");
 
     __shared__ double Bs[2][2][NT * LDB];     // [buffer][row parity]
@@ -72,6 +73,11 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
     const double* srcp[2] = {nullptr, nullptr};
     int v_dst = 0, kk0 = 0;
     bool src_ok = false;
+    // LDIV: column l (parity cp) of the undivided pass-2 output, row k of parity rp,
+    // is divided by lam_rp(k) + lam_cp(l) - xlambda as it is loaded
+    int ld_l = 0, ld_cp = 0;
+    double ld_xlam = 0.0;
+    bool ld_gauge = false;
 
     const d2v* colA[PPT];                     // C2P: column j' and its mirror n-1-j'
     const d2v* colB[PPT];
@@ -114,6 +120,13 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
             const int cp = nt >> 1;
             srcp[0] = a.X + ((size_t)(plane * 4 + cp) * NH + l) * NH;
             srcp[1] = a.X + ((size_t)(plane * 4 + 2 + cp) * NH + l) * NH;
+            if constexpr (LDIV) {
+                const int f = (int)(fm / g.nmodes), mode = (int)(fm % g.nmodes);
+                ld_l = l;
+                ld_cp = cp;
+                ld_xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
+                ld_gauge = a.gauge >= 0 && fabs(ld_xlam) <= 1.0e-10 && cp == 0 && l == a.gauge;
+            }
         }
     }
 
@@ -145,6 +158,17 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
                     breg[rp][e] = v[0];
                     breg[rp][e + 1] = v[1];
                 }
+            if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
+                const int hh[2] = {he, n >> 1};
+#pragma unroll
+                for (int rp = 0; rp < 2; ++rp)
+#pragma unroll
+                    for (int e = 0; e < DPT; ++e) {
+                        const int k = k0 + e;
+                        if (ld_l < hh[ld_cp] && k < hh[rp] && !(ld_gauge && rp == 0 && k == a.gauge))
+                            breg[rp][e] = breg[rp][e] / (a.lam[rp * NH + k] + a.lam[ld_cp * NH + ld_l] - ld_xlam);
+                    }
+            }
         }
     };
     auto store_b = [&](int buf) {
@@ -289,9 +313,11 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
             }
     } else {
         double xlam = 0.0;
+        bool gauge_plane = false;
         if constexpr (MODE == PROJ_P2P_DIV) {
             const int f = (int)((fm) / g.nmodes), mode = (int)(fm % g.nmodes);
             xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
+            gauge_plane = a.gauge >= 0 && fabs(xlam) <= 1.0e-10;   // solvediag.f90:161-206
         }
         const int hh[2] = {he, n >> 1};
 #pragma unroll
@@ -307,7 +333,8 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
                     const int row = rbase + (mt % TH) * 16 + 4 * r;
                     double v = acc[mt][nt][r];
                     if constexpr (MODE == PROJ_P2P_DIV) {
-                        if (l < hh[cp] && row < hh[rp])
+                        if (l < hh[cp] && row < hh[rp] &&
+                            !(gauge_plane && cp == 0 && rp == 0 && l == a.gauge && row == a.gauge))
                             v = v / (a.lam[cp * NH + l] + a.lam[rp * NH + row] - xlam);
                         else
                             v = 0.0;
@@ -331,6 +358,7 @@ static hipError_t launch_f(int mode, const ProjFoldArgs& a, hipStream_t st)
         case PROJ_P2P: hipLaunchKernelGGL((k_projf<NP2, PROJ_P2P>), grid, block, 0, st, a); break;
         case PROJ_P2P_DIV: hipLaunchKernelGGL((k_projf<NP2, PROJ_P2P_DIV>), grid, block, 0, st, a); break;
         case PROJ_P2C: hipLaunchKernelGGL((k_projf<NP2, PROJ_P2C>), grid, block, 0, st, a); break;
+        case PROJ_P2P_LDIV: hipLaunchKernelGGL((k_projf<NP2, PROJ_P2P_LDIV>), grid, block, 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -357,7 +385,7 @@ int fold_size(int n)
 }
 
 int build_fold_bases(const double* P, const double* Pinv, const double* lam, int n, int NH,
-                     double* fwd, double* bwd, double* lamf)
+                     double* fwd, double* bwd, double* lamf, int* pos)
 {
     const int he = (n + 1) / 2, ho = n / 2;
     std::vector<int> ev, od;
@@ -371,6 +399,10 @@ int build_fold_bases(const double* P, const double* Pinv, const double* lam, int
         (se > so ? ev : od).push_back(k);
     }
     if ((int)ev.size() != he || (int)od.size() != ho) return -1;
+    if (pos) {
+        for (int r = 0; r < he; ++r) pos[ev[r]] = r;
+        for (int r = 0; r < ho; ++r) pos[od[r]] = -1 - r;
+    }
     const int R = 2 * NH;
     std::vector<double> Af((size_t)R * NH, 0.0), Ab((size_t)R * NH, 0.0);   // column-major, lda = R
     for (int c = 0; c < 2; ++c) {

@@ -147,9 +147,10 @@ def solvediag(U: np.ndarray, bound: str, xlambda: float) -> np.ndarray:
     return out
 
 
-def set_fold(on: bool) -> None:
-    """Dirichlet solves in the even/odd folded eigenbasis (duct_set_fold_)."""
-    _lib.call("duct_set_fold_", _lib.byref_int(1 if on else 0))
+def set_fold(mode) -> None:
+    """Even/odd folded eigenbasis for the FP64 solves (duct_set_fold_):
+    0/False off, 1/True Dirichlet, 2 Dirichlet and Neumann."""
+    _lib.call("duct_set_fold_", _lib.byref_int(int(mode)))
 
 
 def fold_size(bound: str) -> int:

@@ -183,13 +183,16 @@ int duct_mat_invdiag_dev_(const int *Nx, const int *Ny, const double *xlam, cons
 
 /* ---- even/odd folded Dirichlet solve ------------------------------------ */
 
-/* 1: Dirichlet solves (duct_helm3d_dev_, _batch_dev_, duct_helm3d_,
- * duct_int2d3d_, duct_solvediag_) run in the even/odd folded eigenbasis, half
- * the FLOPs of the four products; 0 (default): the reference's unfolded
- * products (solvediag.f90:51-80).  D2B commutes with the reflection of the
- * CGL nodes, so PD's columns are even or odd; the folded solve uses their
- * parity projections and agrees with the unfolded one to ~1e-13 relative at
- * m = 257 (rounding of the reference's nearly-degenerate eigenvector pairs).
+/* Even/odd folded eigenbasis for the FP64 solves (duct_helm3d_dev_, _batch_dev_,
+ * duct_helm3d_, duct_int2d3d_, duct_solvediag_): half the FLOPs of the four
+ * products.  D2B commutes with the reflection of the CGL nodes, so the
+ * columns of PD / PN are even or odd; the folded solve uses their parity
+ * projections.  mode 0 (default): the reference's unfolded products
+ * (solvediag.f90:51-80, :140-219); 1: Dirichlet folded, agreeing with the
+ * unfolded solve to ~1e-13 relative at m = 257; 2: Neumann folded as well,
+ * including the k=0 gauge (:161-206).  The Neumann Poisson planes are
+ * ill-conditioned: there the fold moves the result by up to ~4e-11 relative,
+ * the size of the unfolded solve's own rounding error on those planes.
  * The reference has no such switch.                                         */
 int duct_set_fold_(const int *mode);
 
