@@ -150,7 +150,8 @@ def main():
         return 0.0
 
     res = {}
-    for path in paths:
+    # --path X times only X (a profiled run then holds only X's kernels)
+    for path in (paths if args.path == "auto" else (args.path,)):
         for _ in range(args.warmup):
             steps[path]()
         torch.cuda.synchronize()
@@ -225,14 +226,16 @@ def main():
                                 "resid": "k_resid (FP64 MFMA residual)"}.get(dom, dom),
                      "flop_per_launch": flop_per_launch, "avg_launch_us": avg_launch_s * 1e6},
         "paths": res,
-        "mixed_vs_fp64": res["mixed"]["mode_solves_per_s"] / res["fp64"]["mode_solves_per_s"],
+        "mixed_vs_fp64": (res["mixed"]["mode_solves_per_s"] / res["fp64"]["mode_solves_per_s"]
+                          if "mixed" in res and "fp64" in res else None),
         "fp64_mfma_frac": None,
         "cpu_baseline": cpu,
     }
-    kf = res["fp64"]["kernels"]
-    proj = [k for k in kf if k.startswith("proj_")]
-    t = sum(kf[k]["ms_per_step"] for k in proj) / 1e3
-    out["fp64_mfma_frac"] = (4 * 2.0 * n ** 3 * planes / t) / PEAK["f64"] if t else None
+    if "fp64" in res:
+        kf = res["fp64"]["kernels"]
+        proj = [k for k in kf if k.startswith("proj_")]
+        t = sum(kf[k]["ms_per_step"] for k in proj) / 1e3
+        out["fp64_mfma_frac"] = (4 * 2.0 * n ** 3 * planes / t) / PEAK["f64"] if t else None
     print(json.dumps(out), flush=True)
     duct_amd.shutdown()
 
