@@ -34,6 +34,7 @@ SIGNATURES = {
     "duct_helm3d_adjoint_batch_dev_": (_i, [_pi, _psz, _psz, _pc, _pd, _pi, _pi, _pi, _pd, _sz]),
     "duct_colloc_deriv_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _pd, _pd, _sz]),
     "duct_derivx_dev_": (_i, [_psz, _psz, _pi, _pi, _pi, _pd, _pd, _pd]),
+    "duct_advection_dev_": (_i, [_psz, _psz, _psz, _psz, _psz, _psz, _pi, _pi, _pi, _pd, _pd]),
     "duct_transpose_dev_": (_i, [_pi, _pi, _pi, _pd, _psz, _psz]),
     "duct_cxyz2yzx_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),
     "duct_cyzx2xyz_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),
@@ -49,7 +50,7 @@ STATUS = {0: "OK", 1: "DUCT_ERR_ARG", 2: "DUCT_ERR_NOT_SETUP", 3: "DUCT_ERR_HIP"
 
 PROF_CLASSES = ["proj_c2p", "proj_p2p_div", "gauge", "proj_p2p", "proj_p2c", "boundary", "bc_correct",
                 "transpose", "invdiag", "lp_c2p", "lp_p2p_div", "lp_p2p", "lp_p2c", "resid", "deriv", "adjoint",
-                "fold_c2p", "fold_p2p_div", "fold_p2p", "fold_p2c"]
+                "fold_c2p", "fold_p2p_div", "fold_p2p", "fold_p2c", "xprod"]
 
 
 class DuctError(RuntimeError):

@@ -1,0 +1,23 @@
+// Advection-term products on the matrix cores (declarations).
+#pragma once
+#include <vector>
+
+#include "common.h"
+
+namespace duct {
+
+constexpr int kXprodMaxM = 64;    // complex x-modes kept (2/3 rule: mx1 + 1)
+constexpr int kXprodMaxX = 192;   // physical x points (mgalx)
+
+struct XprodArgs {
+    const d2v* in[3];      // u, v, w: complex(8) (m, m, M)
+    d2v* out[6];           // uu, vv, ww, uv, uw, vw: complex(8) (m, m, M)
+    const double* Einv;    // inverse x-DFT, B-operand fragment layout (2M x X)
+    const double* Fwd;     // forward x-DFT with 1/X, B-operand fragment layout (X x 2M)
+    int m, M, X;
+};
+// M % 16 == 0, M <= 64, X % 16 == 0, X <= 192.
+hipError_t launch_xprod(const XprodArgs& a, hipStream_t st);
+void build_xdft(int M, int X, std::vector<double>& einv, std::vector<double>& fwd);
+
+}  // namespace duct

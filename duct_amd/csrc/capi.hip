@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/duct_hip.h"
+#include "advect.h"
 #include "aux.h"
 #include "common.h"
 #include "proj.h"
@@ -60,7 +61,7 @@ struct ProfRec {
 enum ProfKind { PK_C2P = 0, PK_P2P_DIV = 1, PK_GAUGE = 2, PK_P2P = 3, PK_P2C = 4, PK_BOUNDARY = 5, PK_BCCORR = 6,
                 PK_TRANSPOSE = 7, PK_INVDIAG = 8, PK_LP_C2P = 9, PK_LP_P2P_DIV = 10, PK_LP_P2P = 11,
                 PK_LP_P2C = 12, PK_RESID = 13, PK_DERIV = 14, PK_ADJ = 15, PK_F_C2P = 16,
-                PK_F_P2P_DIV = 17, PK_F_P2P = 18, PK_F_P2C = 19 };
+                PK_F_P2P_DIV = 17, PK_F_P2P = 18, PK_F_P2C = 19, PK_XPROD = 20 };
 
 struct Lib {
     bool init = false;
@@ -77,6 +78,10 @@ struct Lib {
     double* dcfrag[2] = {nullptr, nullptr}; // DC and DC^T, FP64 fragment layout (derivative products)
     int dc_m = 0, dc_np = 0;
     int fold = 0;                         // 1: Dirichlet groups use the folded passes, 2: Neumann too
+    double* xdft[2] = {nullptr, nullptr}; // advection x-DFTs (inverse, forward), fragment layout
+    int xdft_M = 0, xdft_X = 0;
+    d2v* prods = nullptr;                 // the six product fields of the advection term
+    size_t prods_bytes = 0;
     size_t stats_bytes = 0;
     bool prof = false;
     std::vector<ProfRec> recs;
@@ -662,6 +667,11 @@ int duct_shutdown_(void)
     free_ptr(L.dcfrag[0]);
     free_ptr(L.dcfrag[1]);
     L.dc_m = L.dc_np = 0;
+    free_ptr(L.xdft[0]);
+    free_ptr(L.xdft[1]);
+    L.xdft_M = L.xdft_X = 0;
+    free_ptr(L.prods);
+    L.prods_bytes = 0;
     L.plan_bytes = L.hfield_bytes = L.rfield_bytes = L.stats_bytes = 0;
     for (auto& r : L.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto& e : L.pool) (void)hipEventDestroy(e);
@@ -1144,6 +1154,84 @@ int duct_helm3d_adjoint_batch_dev_(const int* nfields, const duct_devptr_t* solb
         }
         if (g.nf == 0) continue;
         if ((st = run_group_adj(g, bidx))) return st;
+    }
+    return OK;
+}
+
+int duct_advection_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const duct_devptr_t* w,
+                        const duct_devptr_t* adv1, const duct_devptr_t* adv2, const duct_devptr_t* adv3,
+                        const int* m, const int* nmodes, const int* mgalx, const double* alp, const double* aspect)
+{
+    if (!u || !v || !w || !adv1 || !adv2 || !adv3 || !m || !nmodes || !mgalx || !alp || !aspect)
+        return fail(ERR_ARG, "null argument to duct_advection_dev_");
+    const duct_devptr_t in[3] = {*u, *v, *w}, out[3] = {*adv1, *adv2, *adv3};
+    for (int i = 0; i < 3; ++i) {
+        if (!in[i] || !out[i]) return fail(ERR_ARG, "null device pointer");
+        for (int j = 0; j < 3; ++j)
+            if (out[i] == in[j]) return fail(ERR_ARG, "adv%d aliases an input velocity", i + 1);
+    }
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    const int M = *nmodes, X = *mgalx, mm = *m;
+    if (M % 16 || M > kXprodMaxM || X % 16 || X > kXprodMaxX || X < 3 * M)
+        return fail(ERR_UNSUPPORTED, "advection needs nmodes %% 16 == 0, nmodes <= %d, mgalx %% 16 == 0, "
+                    "3 nmodes <= mgalx <= %d (got %d, %d)", kXprodMaxM, kXprodMaxX, M, X);
+    if (!(*aspect > 0.0)) return fail(ERR_ARG, "aspect must be positive");
+    if (!L.dcfrag[0] || L.dc_m != mm)
+        return fail(ERR_NOT_SETUP, "no collocation matrix for m=%d (duct_setup_ builds it)", mm);
+    if (L.xdft_M != M || L.xdft_X != X) {
+        std::vector<double> ei, fw;
+        build_xdft(M, X, ei, fw);
+        HIPCHK(hipStreamSynchronize(L.stream));
+        free_ptr(L.xdft[0]);
+        free_ptr(L.xdft[1]);
+        L.xdft_M = L.xdft_X = 0;
+        HIPCHK(hipMalloc((void**)&L.xdft[0], ei.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(L.xdft[0], ei.data(), ei.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc((void**)&L.xdft[1], fw.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(L.xdft[1], fw.data(), fw.size() * sizeof(double), hipMemcpyHostToDevice));
+        L.xdft_M = M;
+        L.xdft_X = X;
+    }
+    const size_t field = (size_t)M * mm * mm;
+    if ((st = ensure_buf(L.prods, L.prods_bytes, 6 * field * sizeof(d2v)))) return st;
+    XprodArgs xa{};
+    for (int i = 0; i < 3; ++i) xa.in[i] = (const d2v*)in[i];
+    for (int p = 0; p < 6; ++p) xa.out[p] = L.prods + p * field;
+    xa.Einv = L.xdft[0];
+    xa.Fwd = L.xdft[1];
+    xa.m = mm; xa.M = M; xa.X = X;
+    {
+        Timed t(PK_XPROD);
+        HIPCHK(launch_xprod(xa, L.stream));
+    }
+    // adv1 = d(uu)/dx + d(uv)/dy + d(uw)/dz, adv2 = d(vv)/dy + d(uv)/dx + d(vw)/dz,
+    // adv3 = d(ww)/dz + d(uw)/dx + d(vw)/dy (operators.F:91-272)
+    const d2v* P[6];
+    for (int p = 0; p < 6; ++p) P[p] = L.prods + p * field;
+    enum { UU, VV, WW, UV, UW, VW };
+    struct Term { int out, prod; char dir; };
+    const Term terms[9] = {{0, UU, 'x'}, {0, UV, 'y'}, {0, UW, 'z'}, {1, VV, 'y'}, {1, UV, 'x'}, {1, VW, 'z'},
+                           {2, WW, 'z'}, {2, UW, 'x'}, {2, VW, 'y'}};
+    for (int t = 0; t < 9; ++t) {
+        const Term& T = terms[t];
+        const double beta = (t % 3) ? 1.0 : 0.0;
+        d2v* o = (d2v*)out[T.out];
+        Timed tm(PK_DERIV);
+        if (T.dir == 'x') {
+            HIPCHK(launch_derivx(P[T.prod], o, mm, M, 0, *alp, 1.0, beta, L.stream));
+        } else {
+            CollocArgs a{};
+            a.g.nf = 1; a.g.nmodes = M; a.g.m = mm; a.g.n = mm - 2;
+            a.g.in[0] = P[T.prod];
+            a.g.out[0] = o;
+            a.Dfrag = L.dcfrag[0];
+            a.NP = L.dc_np;
+            a.scal = T.dir == 'z' ? 1.0 / *aspect : 1.0;
+            a.beta = beta;
+            HIPCHK(launch_colloc(T.dir == 'y' ? CL_DY : CL_DZ, a, L.stream));
+        }
     }
     return OK;
 }

@@ -9,7 +9,9 @@ module duct_hip
   implicit none
   private
   public :: duct_init, duct_shutdown, duct_setup, duct_helm3d, duct_helm3d_dev, &
-            duct_helm3d_batch_dev, duct_int2d3d, duct_solvediag, duct_check
+            duct_helm3d_batch_dev, duct_int2d3d, duct_solvediag, duct_check, duct_set_fold, &
+            duct_helm3d_mixed_batch_dev, duct_helm3d_adjoint_batch_dev, duct_colloc_deriv_dev, &
+            duct_derivx_dev, duct_advection_dev
 
   interface
     integer(c_int) function duct_init(device) bind(C, name="duct_init_")
@@ -91,6 +93,63 @@ module duct_hip
       character(kind=c_char), intent(in) :: bound
       real(c_double), intent(in) :: xlambda
       integer(c_size_t), value :: bound_len
+    end function
+
+    ! even/odd folded eigenbasis for the FP64 solves: 0 off (default), 1 Dirichlet, 2 both
+    integer(c_int) function duct_set_fold(mode) bind(C, name="duct_set_fold_")
+      import :: c_int
+      integer(c_int), intent(in) :: mode
+    end function
+
+    ! mixed-precision refined variant of duct_helm3d_batch_dev
+    integer(c_int) function duct_helm3d_mixed_batch_dev(nfields, rhs, sol, bounds, alambda, m, nmodes, ib, &
+        alp, scheme0, scheme1, niter, stats, bounds_len) bind(C, name="duct_helm3d_mixed_batch_dev_")
+      import :: c_int, c_double, c_size_t, c_char, c_ptr
+      integer(c_int), intent(in) :: nfields, m, nmodes, ib, scheme0, scheme1, niter
+      integer(c_size_t), intent(in) :: rhs(*), sol(*)
+      character(kind=c_char), intent(in) :: bounds(*)
+      real(c_double), intent(in) :: alambda(*), alp
+      type(c_ptr), value :: stats
+      integer(c_size_t), value :: bounds_len
+    end function
+
+    ! transposed solve (the backward of helm3d)
+    integer(c_int) function duct_helm3d_adjoint_batch_dev(nfields, solbar, rhsbar, bounds, alambda, m, nmodes, &
+        ib, alp, bounds_len) bind(C, name="duct_helm3d_adjoint_batch_dev_")
+      import :: c_int, c_double, c_size_t, c_char
+      integer(c_int), intent(in) :: nfields, m, nmodes, ib
+      integer(c_size_t), intent(in) :: solbar(*), rhsbar(*)
+      character(kind=c_char), intent(in) :: bounds(*)
+      real(c_double), intent(in) :: alambda(*), alp
+      integer(c_size_t), value :: bounds_len
+    end function
+
+    ! collocation derivative products along y or z, and d/dx in Fourier space
+    integer(c_int) function duct_colloc_deriv_dev(in, out, m, nmodes, cdir, trans, scal, beta, cdir_len) &
+        bind(C, name="duct_colloc_deriv_dev_")
+      import :: c_int, c_double, c_size_t, c_char
+      integer(c_size_t), intent(in) :: in, out
+      integer(c_int), intent(in) :: m, nmodes, trans
+      character(kind=c_char), intent(in) :: cdir
+      real(c_double), intent(in) :: scal, beta
+      integer(c_size_t), value :: cdir_len
+    end function
+
+    integer(c_int) function duct_derivx_dev(in, out, m, nmodes, ib, alp, scal, beta) &
+        bind(C, name="duct_derivx_dev_")
+      import :: c_int, c_double, c_size_t
+      integer(c_size_t), intent(in) :: in, out
+      integer(c_int), intent(in) :: m, nmodes, ib
+      real(c_double), intent(in) :: alp, scal, beta
+    end function
+
+    ! advection3d (operators.F:9-276) on device fields
+    integer(c_int) function duct_advection_dev(u, v, w, adv1, adv2, adv3, m, nmodes, mgalx, alp, aspect) &
+        bind(C, name="duct_advection_dev_")
+      import :: c_int, c_double, c_size_t
+      integer(c_size_t), intent(in) :: u, v, w, adv1, adv2, adv3
+      integer(c_int), intent(in) :: m, nmodes, mgalx
+      real(c_double), intent(in) :: alp, aspect
     end function
   end interface
 

@@ -10,6 +10,11 @@ Generator (the same for every configuration):
   (pseudomat.f90:83-85); the k=0 mode keeps only its real part.
 Dirichlet fields get their boundary overwritten with 0 (helm3d.f90:48-60 does
 the same); Neumann solves read only the interior.
+
+Config 3's velocity (duct_velocity) is the laminar square-duct profile plus a
+divergence-free perturbation curl(A), A_c = (1-y^2)^2 (1-z^2)^2 h_c with h_c a
+seeded Chebyshev series, coefficients (g1 + i g2) exp(-0.2 (p + q + |k|)),
+scaled to 5% of the laminar maximum; seeds seed_for(3, c) for A_x, A_y, A_z.
 """
 from __future__ import annotations
 
@@ -84,3 +89,57 @@ def rhs_field(cfg: Config, field: int, m: int | None = None, nmodes: int | None 
 
 def alambda_of(cfg: Config, field: int) -> float:
     return cfg.alambda_dir if BOUNDS[field] == "d" else cfg.alambda_neu
+
+
+def laminar_duct(m: int) -> np.ndarray:
+    """Laminar square-duct profile U(z, y): (D_y^2 + D_z^2) U = -1, U = 0 on the walls,
+    scaled to max U = 1 (the flow the reference's dpval table describes, operators.F:2486-2507)."""
+    from .basis import collocation_matrix
+    _, DC = collocation_matrix(m)
+    D2 = (DC @ DC)[1:-1, 1:-1]
+    lam, P = np.linalg.eig(D2)
+    lam, P = lam.real, P.real
+    Pi = np.linalg.inv(P)
+    F = -np.ones((m - 2, m - 2))
+    G = (Pi @ F @ Pi.T) / (lam[:, None] + lam[None, :])
+    U = np.zeros((m, m))
+    U[1:-1, 1:-1] = P @ G @ P.T
+    return U / U.max()
+
+
+def duct_velocity(cfg: Config, m: int | None = None, nmodes: int | None = None, amp: float = 0.05,
+                  kdeg: int = 24):
+    """u, v, w complex128 (nmodes, m, m) [mode, z, y] of the pseudo-spectral step (config 3)."""
+    from numpy.polynomial import chebyshev as C
+    m = cfg.m if m is None else m
+    M = cfg.nmodes if nmodes is None else nmodes
+    K = min(kdeg, m - 5)                         # g(y) h(y) stays of degree <= m-1
+    x = -np.cos(np.pi * np.arange(m) / (m - 1))
+    T = cheb_nodes_matrix(m, K - 1)              # T_p(x_j), [node, p]
+    dT = np.stack([C.chebval(x, C.chebder(np.eye(K)[p])) for p in range(K)], axis=1)
+    g, dg = (1 - x ** 2) ** 2, -4.0 * x * (1 - x ** 2)
+    k = np.arange(M)
+    decay = np.exp(-0.2 * (np.arange(K)[:, None] + np.arange(K)[None, :]))
+    A, Ay, Az = [], [], []
+    for c in range(3):
+        rng = np.random.Generator(np.random.PCG64(seed_for(cfg.index, c)))
+        a = (rng.standard_normal((M, K, K)) + 1j * rng.standard_normal((M, K, K))) * decay[None]
+        a *= np.exp(-0.2 * k)[:, None, None]
+        a[0] = a[0].real
+        H = np.einsum("lq,kqp,jp->klj", T, a, T, optimize=True)       # [k, z, y]
+        Hy = np.einsum("lq,kqp,jp->klj", T, a, dT, optimize=True)
+        Hz = np.einsum("lq,kqp,jp->klj", dT, a, T, optimize=True)
+        gz, gy = g[None, :, None], g[None, None, :]
+        A.append(gz * gy * H)
+        Ay.append(gz * (dg[None, None, :] * H + gy * Hy))
+        Az.append(gy * (dg[None, :, None] * H + gz * Hz))
+    ikx = (1j * cfg.alp * k)[:, None, None]
+    up = Ay[2] - Az[1]                            # u' = dA_z/dy - dA_y/dz
+    vp = Az[0] - ikx * A[2]                       # v' = dA_x/dz - dA_z/dx
+    wp = ikx * A[1] - Ay[0]                       # w' = dA_y/dx - dA_x/dy
+    eps = amp / max(np.abs(up).max(), np.abs(vp).max(), np.abs(wp).max())
+    u, v, w = eps * up, eps * vp, eps * wp
+    u[0] += laminar_duct(m)
+    for f in (u, v, w):
+        f[0] = f[0].real
+    return [np.ascontiguousarray(f) for f in (u, v, w)]

@@ -158,6 +158,24 @@ int duct_derivx_dev_(const duct_devptr_t *in, const duct_devptr_t *out, const in
                      const int *nmodes, const int *ib, const double *alp, const double *scal,
                      const double *beta);
 
+/* Advection term in divergence form (advection3d, src/operators.F:9-276) on
+ * device fields u, v, w: complex(8) (m, m, nmodes), Fourier modes 0..nmodes-1
+ * in x (the whole x spectrum, i.e. the reference's z-cut after chx2z), physical
+ * (CGL) in y and z, y fastest.  Writes
+ *   adv1 = d(uu)/dx + d(uv)/dy + d(uw)/dz,  adv2 = d(uv)/dx + d(vv)/dy + d(vw)/dz,
+ *   adv3 = d(uw)/dx + d(vw)/dy + d(ww)/dz
+ * with the products formed on mgalx physical x points by the 3/2 rule (fourx,
+ * mult, pack_fftw: src/fourier.F:1-115, src/mult.F) and d/dx = i alp k,
+ * d/dy = DC, d/dz = DC / aspect (the collocation form of transyz + derivyz,
+ * src/chebyshev.f90).  Needs duct_setup_ for m; nmodes % 16 == 0,
+ * nmodes <= 64, mgalx % 16 == 0, 3 nmodes <= mgalx <= 192.  The adv fields must
+ * not alias u, v, w.  The reference's slip-velocity, CFL and statistics side
+ * outputs of advection3d are not computed.                                  */
+int duct_advection_dev_(const duct_devptr_t *u, const duct_devptr_t *v, const duct_devptr_t *w,
+                        const duct_devptr_t *adv1, const duct_devptr_t *adv2, const duct_devptr_t *adv3,
+                        const int *m, const int *nmodes, const int *mgalx, const double *alp,
+                        const double *aspect);
+
 /* ---- bandwidth-bound helpers (reference GPU glue) ----------------------- */
 
 /* C = alpha * A^T for nbatch consecutive Nx x Ny FP64 matrices (column-major,
@@ -209,7 +227,7 @@ int duct_prof_enable_(const int *on);
  * 1 proj_p2p_div, 2 gauge, 3 proj_p2p, 4 proj_p2c, 5 boundary, 6 bc_correct,
  * 7 transpose, 8 invdiag, 9 lp_c2p, 10 lp_p2p_div, 11 lp_p2p, 12 lp_p2c,
  * 13 resid, 14 deriv, 15 adjoint (Neumann boundary adjoint), 16 fold_c2p,
- * 17 fold_p2p_div, 18 fold_p2p, 19 fold_p2c. */
+ * 17 fold_p2p_div, 18 fold_p2p, 19 fold_p2c, 20 xprod (advection products). */
 int duct_prof_read_(const int *nclass, double *ms, int *count);
 
 #ifdef __cplusplus

@@ -1,0 +1,148 @@
+"""GPU tests of the advection term (duct_advection_dev_) and of the pseudo-spectral
+step (config 3: 192 x-points, 64 modes, 129^2 CGL points), against the oracle,
+which follows the reference's own route (FFTW-convention x transforms, REDFT00
+Chebyshev transforms and the derivyz recurrence, oracle/spectral.py) where the
+GPU uses dense x-DFTs on the matrix cores and collocation products.
+
+Bars: advection 1e-12 relative (max norm per component); the full step 1e-10
+(it contains the k=0 pressure-Poisson solve, ~1e-11 conditioning at this size).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    import duct_amd
+    torch.cuda.set_device(0)
+    duct_amd.init(0)
+    yield duct_amd
+    duct_amd.shutdown()
+
+
+_cur = {"m": None}
+
+
+def ensure(dev, m):
+    if _cur["m"] != m:
+        dev.setup(m)
+        _cur["m"] = m
+
+
+def cuda(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def random_velocity(m, M, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(3):
+        f = (rng.standard_normal((M, m, m)) + 1j * rng.standard_normal((M, m, m))) * \
+            np.exp(-0.1 * np.arange(M))[:, None, None]
+        f[0] = f[0].real
+        out.append(f)
+    return out
+
+
+def run_adv(dev, fields, mgalx, alp, aspect=1.0):
+    import torch
+    from duct_amd import step
+    u = [cuda(f) for f in fields]
+    adv = [torch.zeros_like(u[0]) for _ in range(3)]
+    step.advection(*u, *adv, mgalx=mgalx, alp=alp, aspect=aspect)
+    torch.cuda.synchronize()
+    return [a.cpu().numpy() for a in adv]
+
+
+@pytest.mark.parametrize("m,M,mgalx,alp", [(17, 16, 48, 1.3), (33, 32, 96, 1.0), (40, 16, 64, 0.7), (129, 64, 192, 1.0)])
+def test_advection_matches_oracle(dev, m, M, mgalx, alp):
+    from oracle import spectral
+    ensure(dev, m)
+    fields = random_velocity(m, M, m + M)
+    got = run_adv(dev, fields, mgalx, alp)
+    ref = spectral.advection3d(*fields, mgalx=mgalx, alp=alp)
+    for c in range(3):
+        e = relerr(got[c], ref[c])
+        print("m=%d M=%d mgalx=%d adv%d rel.err %.2e" % (m, M, mgalx, c + 1, e), flush=True)
+        assert e <= 1e-12
+
+
+def test_advection_aspect_and_errors(dev):
+    from duct_amd._lib import DuctError
+    from oracle import spectral
+    m, M, mgalx = 17, 16, 48
+    ensure(dev, m)
+    fields = random_velocity(m, M, 3)
+    got = run_adv(dev, fields, mgalx, 1.0, aspect=2.0)
+    ref = spectral.advection3d(*fields, mgalx=mgalx, alp=1.0, aspect=2.0)
+    for c in range(3):
+        assert relerr(got[c], ref[c]) <= 1e-12
+    with pytest.raises(DuctError):
+        run_adv(dev, [f[:8] for f in fields], 48, 1.0)             # nmodes % 16 != 0
+    with pytest.raises(DuctError):
+        run_adv(dev, fields, 40, 1.0)                               # mgalx < 3 nmodes
+    import torch
+    from duct_amd import step
+    u = [cuda(f) for f in fields]
+    with pytest.raises(DuctError):
+        step.advection(u[0], u[1], u[2], u[0], torch.zeros_like(u[0]), torch.zeros_like(u[0]), mgalx=48)
+
+
+def test_config3_step(dev, oracle_mod):
+    """Config 3: laminar duct + 5% divergence-free perturbation; advection, three Helmholtz
+    solves, the pressure Poisson solve and the projection, against the oracle."""
+    import torch
+    from duct_amd import step, workloads
+    from oracle import spectral as S
+    cfg = workloads.CONFIGS[3]
+    m, M, mgalx, nu, dt = cfg.m, cfg.nmodes, 192, 0.01, 0.1
+    ensure(dev, m)
+    vel = workloads.duct_velocity(cfg)
+    ut = [cuda(f) for f in vel]
+    un, vn, wn, phi, adv = step.c3_step(*ut, nu=nu, dt=dt, mgalx=mgalx)
+    torch.cuda.synchronize()
+    got = [t.cpu().numpy() for t in (un, vn, wn)]
+    # oracle step
+    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(m))
+    alam = 1.0 / (nu * dt)
+    radv = S.advection3d(*vel, mgalx=mgalx, alp=cfg.alp)
+    for c in range(3):
+        e = relerr(adv[c].cpu().numpy(), radv[c])
+        print("config 3 adv%d rel.err %.2e" % (c + 1, e), flush=True)
+        assert e <= 1e-12
+    oracle_mod.use_blas(True)
+    try:
+        us = [oracle_mod.helm3d(ob, a / nu - alam * x, "d", alam, nthreads=8)[0] for a, x in zip(radv, vel)]
+
+        def dy(f):
+            return S.transyz(S.derivyz(S.transyz(f, "y", -1), "y", 1), "y", +1)
+
+        def dz(f):
+            return S.transyz(S.derivyz(S.transyz(f, "z", -1), "z", 1), "z", +1)
+
+        d = (S.derivx(us[0], cfg.alp) + dy(us[1]) + dz(us[2])) / dt
+        ph = oracle_mod.helm3d(ob, d, "n", 0.0, nthreads=8)[0]
+    finally:
+        oracle_mod.use_blas(False)
+    ref = (us[0] - dt * S.derivx(ph, cfg.alp), us[1] - dt * dy(ph), us[2] - dt * dz(ph))
+    for c in range(3):
+        e = relerr(got[c], ref[c])
+        print("config 3 step u%d rel.err %.2e" % (c + 1, e), flush=True)
+        assert e <= 1e-10
+    # the projected velocity is divergence-free in the interior (collocation)
+    from duct_amd import basis
+    _, DC = basis.collocation_matrix(m)
+    k = np.arange(M)[:, None, None]
+    div = 1j * cfg.alp * k * got[0] + got[1] @ DC.T + np.einsum("ij,mjk->mik", DC, got[2])
+    dstar = d * dt
+    rel = np.abs(div[:, 1:-1, 1:-1]).max() / np.abs(dstar[:, 1:-1, 1:-1]).max()
+    print("config 3 interior divergence after projection / before: %.2e" % rel, flush=True)
+    assert rel <= 1e-9
