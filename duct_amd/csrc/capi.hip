@@ -16,6 +16,7 @@
 #include "advect.h"
 #include "aux.h"
 #include "common.h"
+#include "mixed_fold.h"
 #include "proj.h"
 #include "proj_lp.h"
 
@@ -49,6 +50,15 @@ struct DevBasis {
     double* fprow = nullptr; // Neumann: signed folded rows of P at the gauge node, 2 NH
     double fpdiag = 0.0;     // Neumann: P_e(a', io_e)
     int fio = -1;            // Neumann: index of the null mode among the even eigenvectors
+    // folded mixed solve (NH = 64 or 128): fp16 hi/lo bases, float eigenvalues, folded S blocks
+    bool lq_ready = false;
+    void* lq_fwd[2] = {nullptr, nullptr};
+    void* lq_bwd[2] = {nullptr, nullptr};
+    float lq_sfwd[2] = {1.f, 1.f}, lq_sbwd[2] = {1.f, 1.f};
+    float* lq_lam = nullptr;
+    double* sfold[2] = {nullptr, nullptr};   // centrosymmetric part of D2B, folded e / o, FP64 fragments
+    double* nqf = nullptr;                   // Neumann: folded P^-1 row of the null mode
+    double* npf = nullptr;                   // Neumann: folded P column of the null mode
     double pdiag = 0.0, wanted = 0.0;
     int gauge = -1;          // 0-based eigen-index of the null mode (Neumann)
 };
@@ -82,6 +92,8 @@ struct Lib {
     int xdft_M = 0, xdft_X = 0;
     d2v* prods = nullptr;                 // the six product fields of the advection term
     size_t prods_bytes = 0;
+    float* mfold = nullptr;               // folded FP32 arrays of the folded mixed solve
+    size_t mfold_bytes = 0;
     size_t stats_bytes = 0;
     bool prof = false;
     std::vector<ProfRec> recs;
@@ -105,6 +117,8 @@ void free_basis(DevBasis& B)
     free_ptr(B.lph_bwd); free_ptr(B.opfrag); free_ptr(B.qrow); free_ptr(B.pcol);
     free_ptr(B.AfwdT); free_ptr(B.AbwdT); free_ptr(B.ffwd); free_ptr(B.fbwd); free_ptr(B.flam);
     free_ptr(B.fprow);
+    for (int p = 0; p < 2; ++p) { free_ptr(B.lq_fwd[p]); free_ptr(B.lq_bwd[p]); free_ptr(B.sfold[p]); }
+    free_ptr(B.lq_lam); free_ptr(B.nqf); free_ptr(B.npf);
     B = DevBasis{};
 }
 
@@ -507,6 +521,197 @@ int run_group_lp(const FieldGroup& g, int bidx, int scheme, int accumulate, doub
 //   u = S0(f);  repeat niter times: r = f - L u (FP64), u += S1(r);  boundary (FP64).
 // stats (host, may be null): [(niter+1) x nf x 2] = max|r_it|, max|f| per field,
 // including the residual after the last correction.
+// Folded mixed solve of one boundary group (mixed_fold.hip): fold, fused low-precision
+// solve, niter x (folded FP64 residual, null projection, fused solve), one more
+// correction of the low Poisson modes (|xlambda| < kLowModes), unfold-and-sum,
+// FP64 gauge, boundary.  stats as in solve_fields_mixed (the last entry is the
+// residual after the final correction, computed only when asked for).
+constexpr double kLowModes = 64.0;
+
+int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long long* stats, const int* fidx,
+                         int nf_all)
+{
+    DevBasis& B = L.basis[bidx];
+    const int NH = B.NH;
+    const size_t arr = (size_t)g.nf * g.nmodes * 2 * 4 * NH * NH;   // floats per folded array
+    const int narr = 2 + niter + 1 + 1;                               // F, U0, D1..Dniter, Dlow, R
+    int st = ensure_buf(L.mfold, L.mfold_bytes, arr * narr * sizeof(float));
+    if (st) return st;
+    float* F = L.mfold;
+    float* R = L.mfold + arr;
+    float* D[kMaxFoldSources + 1];
+    for (int t = 0; t < niter + 2; ++t) D[t] = L.mfold + (size_t)(2 + t) * arr;   // D[0] = U0, D[niter+1] = low
+    {
+        Timed t(PK_LP_C2P);
+        HIPCHK(launch_fold32(g, NH, F, L.stream));
+    }
+    LpqArgs la{};
+    for (int p = 0; p < 2; ++p) {
+        la.fwd[p] = B.lq_fwd[p]; la.bwd[p] = B.lq_bwd[p];
+        la.sfwd[p] = B.lq_sfwd[p]; la.sbwd[p] = B.lq_sbwd[p];
+    }
+    la.lam = B.lq_lam;
+    la.gauge = bidx == 1 ? B.fio : -1;
+    auto lpq = [&](const FieldGroup& gg, const float* in, float* out) -> int {
+        Timed t(PK_LP_P2P);
+        la.g = gg; la.X = in; la.Y = out;
+        HIPCHK(launch_lpq(NH, la, L.stream));
+        return OK;
+    };
+    // residual of the modes of gg (a prefix, or a suffix starting at plane offset `off`) with u = sum of D[0..nsrc)
+    auto resid = [&](const FieldGroup& gg, int nsrc, unsigned long long* sp, size_t off) -> int {
+        ResidFoldArgs ra{};
+        ra.g = gg;
+        for (int t = 0; t < nsrc; ++t) ra.src[t] = D[t] + off;
+        ra.nsrc = nsrc;
+        ra.R = R + off;
+        ra.Sfrag[0] = B.sfold[0]; ra.Sfrag[1] = B.sfold[1];
+        ra.stats = sp;
+        for (int i = 0; i < gg.nf; ++i) ra.slot[i] = fidx[i];
+        ra.neumann = bidx == 1;
+        ra.NH = NH;
+        {
+            Timed t(PK_RESID);
+            HIPCHK(launch_resid_fold(ra, L.stream));
+        }
+        if (bidx == 1) {
+            NullFoldArgs np{};
+            np.R = R + off; np.qf = B.nqf; np.pf = B.npf; np.stats = sp; np.NH = NH; np.nf = gg.nf;
+            for (int i = 0; i < gg.nf; ++i) np.slot[i] = fidx[i];
+            GaugeList gl{};
+            for (int k = 0; k < gg.nmodes; ++k)
+                for (int f = 0; f < gg.nf; ++f) {
+                    if (!(std::fabs(xlambda_of(gg.alambda[f], gg.alp, gg.mode0 + k)) <= 1.0e-10)) continue;
+                    for (int part = 0; part < 2; ++part) {
+                        if (part == 1 && gg.mode0 + k == 0) continue;
+                        gl.plane[gl.count++] = 2 * (k * gg.nf + f) + part;
+                        if (gl.count == kMaxGaugePlanes) {
+                            Timed t(PK_GAUGE);
+                            HIPCHK(launch_nullproj_fold(np, gl, L.stream));
+                            gl.count = 0;
+                        }
+                    }
+                }
+            if (gl.count) {
+                Timed t(PK_GAUGE);
+                HIPCHK(launch_nullproj_fold(np, gl, L.stream));
+            }
+        }
+        return OK;
+    };
+    if ((st = lpq(g, F, D[0]))) return st;
+    for (int it = 0; it < niter; ++it) {
+        if ((st = resid(g, it + 1, stats ? stats + (size_t)2 * nf_all * it : nullptr, 0))) return st;
+        if ((st = lpq(g, R, D[it + 1]))) return st;
+    }
+    {
+        Timed t(PK_LP_P2C);
+        const float* src[kMaxFoldSources] = {D[0], niter >= 1 ? D[1] : nullptr, niter >= 2 ? D[2] : nullptr, nullptr};
+        HIPCHK(launch_unfold_sum(g, NH, src, niter + 1, 0, L.stream));
+    }
+    // The refinement above converges to S^-1 f, S the centrosymmetric part of D2B.  On the
+    // ill-conditioned low Poisson modes (xlambda < kLowModes) that differs from D2B^-1 f by up
+    // to ~1e-10, so those modes get one more correction from the residual of D2B itself
+    // (k_colloc on the unfolded FP64 solution, solvediag's own operator).
+    int klow = 0;
+    if (bidx == 1 && niter >= 1) {
+        while (klow < g.nmodes) {
+            bool low = false;
+            for (int f = 0; f < g.nf; ++f)
+                low = low || xlambda_of(g.alambda[f], g.alp, g.mode0 + klow) < kLowModes;
+            if (!low) break;
+            ++klow;
+        }
+    }
+    auto resid_full = [&](const FieldGroup& gg, unsigned long long* sp) -> int {
+        const size_t fb = (size_t)gg.nmodes * gg.m * gg.m;
+        if ((st = ensure_buf(L.rfield, L.rfield_bytes, fb * gg.nf * sizeof(d2v)))) return st;
+        ResidArgs ra{};
+        ra.g = gg;
+        ra.Dfrag = B.opfrag;
+        ra.NP = B.NP;
+        ra.stats = sp;
+        ra.neumann = bidx == 1;
+        for (int i = 0; i < gg.nf; ++i) { ra.r[i] = L.rfield + (size_t)i * fb; ra.slot[i] = fidx[i]; }
+        {
+            Timed t(PK_RESID);
+            HIPCHK(launch_resid(ra, L.stream));
+        }
+        if (bidx == 1) {
+            NullProjArgs np{};
+            np.q = B.qrow; np.p = B.pcol; np.stats = sp;
+            np.nmodes = gg.nmodes; np.m = gg.m; np.n = gg.n;
+            for (int i = 0; i < gg.nf; ++i) { np.r[i] = ra.r[i]; np.slot[i] = fidx[i]; }
+            GaugeList gl{};
+            for (int f = 0; f < gg.nf; ++f)
+                for (int k = 0; k < gg.nmodes; ++k) {
+                    if (!(std::fabs(xlambda_of(gg.alambda[f], gg.alp, gg.mode0 + k)) <= 1.0e-10)) continue;
+                    for (int part = 0; part < 2; ++part) {
+                        if (part == 1 && gg.mode0 + k == 0) continue;
+                        gl.plane[gl.count++] = 2 * (f * gg.nmodes + k) + part;
+                        if (gl.count == kMaxGaugePlanes) {
+                            Timed t(PK_GAUGE);
+                            HIPCHK(launch_nullproj(np, gl, L.stream));
+                            gl.count = 0;
+                        }
+                    }
+                }
+            if (gl.count) {
+                Timed t(PK_GAUGE);
+                HIPCHK(launch_nullproj(np, gl, L.stream));
+            }
+        }
+        return OK;
+    };
+    if (klow > 0) {
+        FieldGroup glow = g;          // g.in = f, g.out = the current solution
+        glow.nmodes = klow;
+        if ((st = resid_full(glow, nullptr))) return st;
+        FieldGroup gr = glow;         // the residual fields as the right-hand side
+        const size_t fb = (size_t)klow * g.m * g.m;
+        for (int i = 0; i < g.nf; ++i) gr.in[i] = L.rfield + (size_t)i * fb;
+        {
+            Timed t(PK_LP_C2P);
+            HIPCHK(launch_fold32(gr, NH, F, L.stream));
+        }
+        if ((st = lpq(gr, F, D[niter + 1]))) return st;
+        Timed t(PK_LP_P2C);
+        const float* s2[1] = {D[niter + 1]};
+        HIPCHK(launch_unfold_sum(glow, NH, s2, 1, 1, L.stream));
+    }
+    if (bidx == 1) {   // gauge node U(iofzero+1, iofzero+1) = wanted on the singular planes
+        GaugeFixArgs gf{};
+        gf.g = g; gf.p = B.pcol; gf.wanted = B.wanted; gf.a = B.gauge;
+        GaugeList gl{};
+        for (int k = 0; k < g.nmodes; ++k)
+            for (int f = 0; f < g.nf; ++f) {
+                if (!(std::fabs(xlambda_of(g.alambda[f], g.alp, g.mode0 + k)) <= 1.0e-10)) continue;
+                for (int part = 0; part < 2; ++part) {
+                    if (part == 1 && g.mode0 + k == 0) continue;
+                    gl.plane[gl.count++] = 2 * (k * g.nf + f) + part;
+                    if (gl.count == kMaxGaugePlanes) {
+                        Timed t(PK_GAUGE);
+                        HIPCHK(launch_gauge_fix(gf, gl, L.stream));
+                        gl.count = 0;
+                    }
+                }
+            }
+        if (gl.count) {
+            Timed t(PK_GAUGE);
+            HIPCHK(launch_gauge_fix(gf, gl, L.stream));
+        }
+    }
+    if (stats && (st = resid_full(g, stats + (size_t)2 * nf_all * niter))) return st;
+    {
+        Timed t(PK_BOUNDARY);
+        BoundaryArgs ba{};
+        ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
+        ba.mode = bidx == 1 ? BND_NEUMANN : BND_DIR_ZERO;
+        HIPCHK(launch_boundary(ba, L.stream));
+    }
+    return OK;
+}
+
 int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* sol, const char* bounds,
                        const double* alambda, int m, int nmodes, int ib, double alp, int scheme0, int scheme1,
                        int niter, double* stats)
@@ -514,9 +719,13 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
     if (nf < 1 || nf > kMaxFields) return fail(ERR_ARG, "nfields=%d outside [1, %d]", nf, kMaxFields);
     int st = check_common(m, nmodes);
     if (st) return st;
+    const bool folded = scheme0 == LP_FOLD16 || scheme1 == LP_FOLD16;
+    if (folded && scheme0 != scheme1)
+        return fail(ERR_ARG, "scheme 5 (folded) is used for both solves or neither");
     for (int sc : {scheme0, scheme1})
-        if (sc < LP_F32 || sc > LP_BF16X1) return fail(ERR_ARG, "precision scheme %d is not one of 1..4", sc);
+        if (sc < LP_F32 || sc > LP_FOLD16) return fail(ERR_ARG, "precision scheme %d is not one of 1..5", sc);
     if (niter < 0 || niter > 16) return fail(ERR_ARG, "niter=%d outside [0, 16]", niter);
+    if (folded && niter > 2) return fail(ERR_ARG, "the folded solve takes niter <= 2 (got %d)", niter);
     const size_t field_bytes = (size_t)nmodes * m * m * sizeof(d2v);
     if ((st = ensure_buf(L.rfield, L.rfield_bytes, field_bytes * nf))) return st;
     const int nstat = stats ? (niter + 1) : 0;
@@ -543,6 +752,13 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
         DevBasis& B = L.basis[bidx];
         if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
         if (g.m != B.m) return fail(ERR_ARG, "m=%d but the basis was set up for m=%d", g.m, B.m);
+        if (folded) {
+            if (!B.lq_ready)
+                return fail(ERR_UNSUPPORTED, "scheme 5 needs a folding basis with ceil((m-2)/2) in (32, 128] "
+                            "(m=%d, bound '%c')", m, bidx ? 'n' : 'd');
+            if ((st = run_group_mixed_fold(g, bidx, niter, nstat ? L.stats : nullptr, fidx, nf))) return st;
+            continue;
+        }
         if ((st = run_group_lp(g, bidx, scheme0, 0, B.wanted))) return st;
         ResidArgs ra{};
         ra.g = g;
@@ -672,6 +888,8 @@ int duct_shutdown_(void)
     L.xdft_M = L.xdft_X = 0;
     free_ptr(L.prods);
     L.prods_bytes = 0;
+    free_ptr(L.mfold);
+    L.mfold_bytes = 0;
     L.plan_bytes = L.hfield_bytes = L.rfield_bytes = L.stats_bytes = 0;
     for (auto& r : L.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto& e : L.pool) (void)hipEventDestroy(e);
@@ -844,6 +1062,62 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
         std::vector<double> opf((size_t)NP * NP);
         build_frag(op.data(), n, n, NP, opf.data());
         if ((st = up(B.opfrag, opf.data(), opf.size()))) return st;
+        if (B.NH == 64 || B.NH == 128) {   // folded mixed solve (LP_FOLD16)
+            const int NH = B.NH, he = (n + 1) / 2, ho = n / 2;
+            std::vector<double> Af((size_t)2 * NH * NH), Ab((size_t)2 * NH * NH), fl((size_t)2 * NH);
+            std::vector<int> pos(n);
+            if (fold_matrices(P, Pinv, lam, n, NH, Af.data(), Ab.data(), fl.data(), pos.data()) == 0) {
+                auto upraw2 = [&](void*& dst, const void* src, size_t bytes) -> int {
+                    HIPCHK(hipMalloc(&dst, bytes));
+                    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+                    return OK;
+                };
+                std::vector<unsigned short> fr;
+                for (int par = 0; par < 2; ++par) {
+                    B.lq_sfwd[par] = build_lpq_basis(Af.data() + (size_t)par * NH, 2 * NH, NH, fr);
+                    if ((st = upraw2(B.lq_fwd[par], fr.data(), fr.size() * 2))) return st;
+                    B.lq_sbwd[par] = build_lpq_basis(Ab.data() + (size_t)par * NH, 2 * NH, NH, fr);
+                    if ((st = upraw2(B.lq_bwd[par], fr.data(), fr.size() * 2))) return st;
+                }
+                std::vector<float> lf(fl.begin(), fl.end());
+                void* lv = nullptr;
+                if ((st = upraw2(lv, lf.data(), lf.size() * sizeof(float)))) return st;
+                B.lq_lam = (float*)lv;
+                // S = (op + J op J)/2 folded: S_e(i,j) = S(i,j) + S(i,n-1-j) (the centre once), S_o with -
+                auto Sv = [&](int i, int j) {
+                    return 0.5 * (op[i + (size_t)j * n] + op[(n - 1 - i) + (size_t)(n - 1 - j) * n]);
+                };
+                std::vector<double> Sb((size_t)NH * NH), sfr((size_t)NH * NH);
+                for (int par = 0; par < 2; ++par) {
+                    const int h = par ? ho : he;
+                    const double sg = par ? -1.0 : 1.0;
+                    std::fill(Sb.begin(), Sb.end(), 0.0);
+                    for (int j = 0; j < h; ++j)
+                        for (int i = 0; i < h; ++i) {
+                            const int jm = n - 1 - j;
+                            Sb[i + (size_t)j * NH] = (j == jm) ? Sv(i, j) : Sv(i, j) + sg * Sv(i, jm);
+                        }
+                    build_frag(Sb.data(), h, NH, NH, sfr.data());
+                    if ((st = up(B.sfold[par], sfr.data(), sfr.size()))) return st;
+                }
+                bool ok = true;
+                if (bidx == 1) {
+                    const int io = *iofzero - 1;
+                    ok = pos[io] >= 0;
+                    if (ok) {
+                        const int fio = pos[io];
+                        std::vector<double> qf(NH), pf(NH);
+                        for (int j = 0; j < NH; ++j) {
+                            qf[j] = Af[(size_t)fio + (size_t)j * 2 * NH];
+                            pf[j] = Ab[(size_t)j + (size_t)fio * 2 * NH];
+                        }
+                        if ((st = up(B.nqf, qf.data(), NH))) return st;
+                        if ((st = up(B.npf, pf.data(), NH))) return st;
+                    }
+                }
+                B.lq_ready = ok;
+            }
+        }
         if (bidx == 1) {
             const int io = *iofzero - 1;
             std::vector<double> q(n), pc(n);

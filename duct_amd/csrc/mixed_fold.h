@@ -1,0 +1,79 @@
+// Folded (even/odd) mixed-precision refined solve: declarations.
+//
+// Intermediates are FP32 "folded planar" arrays: plane p = 2 (f nmodes + mode)
+// + part holds four NH x NH column-major quarters, quarter q = 2 r + c (r the
+// parity of y, c of z), element (y', z') at ((p 4 + q) NH + z') NH + y'.
+#pragma once
+#include <vector>
+
+#include "common.h"
+#include "proj.h"
+
+namespace duct {
+
+constexpr int kMaxFoldSources = 4;
+
+// fold: FP32 quarters of the interior of complex fields g.in (half-sum/half-difference
+// in both directions, /4), zero padding, Im of mode 0 zero.
+hipError_t launch_fold32(const FieldGroup& g, int NH, float* X, hipStream_t st);
+
+// unfold: g.out interior (= or +=) the unfolded sum of nsrc folded FP32 arrays (summed in FP64).
+hipError_t launch_unfold_sum(const FieldGroup& g, int NH, const float* const* src, int nsrc, int accumulate,
+                             hipStream_t st);
+
+// The fused low-precision solve of every quarter: U = P_r ((P_r^-1 X P_c^-T) ./ den) P_c^T
+// with fp16 hi/lo splits (3 products, FP32 accumulation), per-pass power-of-two scaling.
+struct LpqArgs {
+    const void* fwd[2];     // folded P^-1 of parity e / o, fp16 hi/lo fragments (build_lpq_basis)
+    const void* bwd[2];     // folded P
+    float sfwd[2], sbwd[2]; // the bases' power-of-two scales (stored = basis * scale)
+    const float* lam;       // [lam_e | lam_o], NH each
+    const float* X;         // folded FP32 input
+    float* Y;               // folded FP32 output
+    int gauge;              // index of the Neumann null mode among the even eigenvectors, -1 if none
+    FieldGroup g;           // planes, xlambda per plane
+};
+hipError_t launch_lpq(int NH, const LpqArgs& a, hipStream_t st);
+// fp16 hi/lo fragments of an NH x NH block B (row i, column k, column-major with leading dim ld);
+// returns the scale s (B * s has max magnitude in [0.5, 1)).
+float build_lpq_basis(const double* B, int ld, int NH, std::vector<unsigned short>& frag);
+
+// Folded FP64 residual r = f - (S_r u + u S_c^T - xlambda u) on every quarter, u = sum of
+// nsrc folded FP32 arrays (FP64 sum), f folded from the complex fields g.in on load.
+struct ResidFoldArgs {
+    FieldGroup g;                       // g.in = f (complex fields)
+    const float* src[kMaxFoldSources];  // u terms
+    int nsrc;
+    float* R;                           // folded FP32 residual
+    const double* Sfrag[2];             // folded S_e, S_o, FP64 fragment layout (build_frag, NP = NH)
+    unsigned long long* stats;          // max|r| (bit patterns) at stats[2 slot[f]], max|f| at +1, or null
+    int slot[kMaxFields];
+    int neumann;                        // singular planes' max|r| is left to the null projection
+    int NH;
+};
+hipError_t launch_resid_fold(const ResidFoldArgs& a, hipStream_t st);
+
+// Singular (Neumann, |xlambda| <= 1e-10) planes: r_ee -= (qf^T r_ee qf) pf pf^T, the null-space
+// part of the residual removed in FP64; stats = max|r| over the four quarters afterwards.
+struct NullFoldArgs {
+    float* R;
+    const double* qf;                   // folded row of P^-1 for the null mode, NH
+    const double* pf;                   // folded null column of P, NH
+    unsigned long long* stats;
+    int slot[kMaxFields];
+    int NH, nf;
+};
+hipError_t launch_nullproj_fold(const NullFoldArgs& a, const GaugeList& list, hipStream_t st);
+
+// Gauge on the unfolded FP64 solution of singular planes: u += c p p^T with
+// c = (wanted - u(a,a)) / p(a)^2, p the null column of P (solvediag.f90:161-206 sets
+// the same node to the same value).
+struct GaugeFixArgs {
+    FieldGroup g;                       // g.out = solution fields
+    const double* p;                    // P(:, iofzero), n
+    double wanted;
+    int a;                              // 0-based interior gauge node (iofzero - 1)
+};
+hipError_t launch_gauge_fix(const GaugeFixArgs& a, const GaugeList& list, hipStream_t st);
+
+}  // namespace duct

@@ -1,0 +1,657 @@
+// Folded (even/odd) mixed-precision refined solve.
+//
+// The refinement of the reference system L u = f (solvediag.f90; L = D2B_y +
+// D2B_z - xlambda) runs in the parity-folded basis of proj_fold.hip:
+//   F  = fold(f)                       FP32 quarters                 k_fold32
+//   U0 = S~(F)                         fused low-precision solve      k_lpq
+//   R  = f - S U0                      FP64 residual, folded          k_resid_fold (+ k_nullproj_fold)
+//   D1 = S~(R)                         fused low-precision solve      k_lpq
+//   u  = unfold(U0 + D1)               FP64                           k_unfold_sum (+ k_gauge_fix)
+// S is the centrosymmetric part (D2B + J D2B J)/2 of the interior operator: its
+// folded blocks are exactly block-diagonal, so the residual costs two n^3/2
+// products per plane.  D2B itself is centrosymmetric up to rounding (~2e-15 of
+// its norm); the converged solutions of S and of D2B lie within 2e-11 of each
+// other and both within 2e-11 of the FP64 oracle on the Poisson planes (the
+// rounding level of those ill-conditioned planes), so the refinement targets S.
+// U0 and D1 are FP32: an FP32-rounded U0 is an exact FP64 iterate, and D1 is a
+// correction whose own relative accuracy only needs to match the solve's.
+//
+// Plane order in every folded array: p = 2 (mode nf + f) + part, so the first K
+// modes of all fields are a prefix (a second refinement of the low Poisson
+// modes reuses the same arrays).
+#include <cmath>
+#include <cstring>
+
+#include "mixed_fold.h"
+
+namespace duct {
+
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void plane_decode(const FieldGroup& g, long fmp, int& f, int& mode)
+{
+    mode = (int)(fmp / g.nf);
+    f = (int)(fmp % g.nf);
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fold32(FieldGroup g, int NH, float* X)
+{
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    const long per = (long)NH * NH;
+    if (idx >= (long)g.nf * g.nmodes * per) return;
+    const long fmp = idx / per;
+    const int zq = (int)((idx % per) / NH), yq = (int)(idx % NH);
+    int f, mode;
+    plane_decode(g, fmp, f, mode);
+    const int n = g.n, m = g.m, he = (n + 1) / 2;
+    d2v a = {0.0, 0.0}, b = a, c = a, d = a;
+    if (yq < he && zq < he) {
+        const d2v* base = g.in[f] + (size_t)mode * m * m;
+        const int y0 = yq + 1, y1 = n - yq, z0 = zq + 1, z1 = n - zq;
+        a = base[y0 + (size_t)z0 * m];
+        b = base[y1 + (size_t)z0 * m];
+        c = base[y0 + (size_t)z1 * m];
+        d = base[y1 + (size_t)z1 * m];
+    }
+    const d2v s0 = a + b, d0 = a - b, s1 = c + d, d1 = c - d;
+    const d2v q[4] = {0.25 * (s0 + s1), 0.25 * (s0 - s1), 0.25 * (d0 + d1), 0.25 * (d0 - d1)};
+    const bool zim = (g.mode0 + mode) == 0;
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+        const long p = 2 * fmp + part;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            X[((size_t)(p * 4 + k) * NH + zq) * NH + yq] = (part && zim) ? 0.f : (float)q[k][part];
+    }
+}
+
+hipError_t launch_fold32(const FieldGroup& g, int NH, float* X, hipStream_t st)
+{
+    const long total = (long)g.nf * g.nmodes * NH * NH;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fold32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, g, NH, X);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+struct UnfoldArgs {
+    FieldGroup g;
+    const float* src[kMaxFoldSources];
+    int nsrc, NH, accumulate;
+};
+
+__global__ __launch_bounds__(256) void k_unfold_sum(UnfoldArgs a)
+{
+    const FieldGroup& g = a.g;
+    const int NH = a.NH, n = g.n, m = g.m, he = (n + 1) / 2;
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    const long per = (long)he * he;
+    if (idx >= (long)g.nf * g.nmodes * per) return;
+    const long fmp = idx / per;
+    const int zq = (int)((idx % per) / he), yq = (int)(idx % he);
+    int f, mode;
+    plane_decode(g, fmp, f, mode);
+    d2v V[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        V[k] = d2v{0.0, 0.0};
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+            const size_t o = ((size_t)((2 * fmp + part) * 4 + k) * NH + zq) * NH + yq;
+            double s = 0.0;
+            for (int t = 0; t < a.nsrc; ++t) s += (double)a.src[t][o];
+            V[k][part] = s;
+        }
+    }
+    const bool zim = (g.mode0 + mode) == 0;
+    // V[2 r + c]: r = parity of y, c = parity of z; the mirrored y (z) flips the odd-r (odd-c) terms
+    const d2v e0 = V[0] + V[1], e1 = V[0] - V[1], o0 = V[2] + V[3], o1 = V[2] - V[3];
+    const int y0 = yq + 1, y1 = n - yq, z0 = zq + 1, z1 = n - zq;
+    d2v* out = g.out[f] + (size_t)mode * m * m;
+    auto put = [&](int y, int z, d2v v) {
+        if (zim) v[1] = 0.0;
+        d2v* p = out + y + (size_t)z * m;
+        if (a.accumulate) v += *p;
+        *p = v;
+    };
+    put(y0, z0, e0 + o0);
+    if (y1 != y0) put(y1, z0, e0 - o0);
+    if (z1 != z0) {
+        put(y0, z1, e1 + o1);
+        if (y1 != y0) put(y1, z1, e1 - o1);
+    }
+}
+
+hipError_t launch_unfold_sum(const FieldGroup& g, int NH, const float* const* src, int nsrc, int accumulate,
+                             hipStream_t st)
+{
+    if (nsrc < 1 || nsrc > kMaxFoldSources) return hipErrorInvalidValue;
+    UnfoldArgs a{};
+    a.g = g;
+    for (int t = 0; t < nsrc; ++t) a.src[t] = src[t];
+    a.nsrc = nsrc;
+    a.NH = NH;
+    a.accumulate = accumulate;
+    const long he = (g.n + 1) / 2;
+    const long total = (long)g.nf * g.nmodes * he * he;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unfold_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fused low-precision solve of one quarter per workgroup.
+//
+// Each pass computes Y = (B X)^T for the pass's basis B with the data X as the
+// MFMA A operand (A[j][k] = X[k][j]) and B as the B operand (Bop[k][i] = B[i][k]),
+// so D[j][i] = Y[j][i] lands with four consecutive j per lane: stored column by
+// column (column i, rows j) it is the next pass's X without a transpose.  X lives
+// in LDS as two fp16 terms (hi = fp16(x s), lo = fp16((x s - hi) 2048), s a power
+// of two per pass); the products hi*hi go to one FP32 accumulator, hi*lo + lo*hi
+// to another (weight 1/2048).  Lane layout of v_mfma_f32_16x16x32_f16:
+// A[i][k] lane i + 16 (k/8) element k%8, B[k][j] lane j + 16 (k/8) element k%8,
+// D[i][j] lane j + 16 (i/4) element i%4.
+// Passes: 0 B = P_r^-1, 1 B = P_c^-1 then ./ (lam_r + lam_c - xlambda),
+// 2 B = P_r, 3 B = P_c -> U(y', z').
+template <int NH>
+__global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
+{
+    constexpr int LDX = NH + 8;          // halves per LDS column
+    constexpr int TJ = NH / 32;          // MFMA tiles per wave along j (2 waves along j)
+    constexpr int TI = NH / 64;          // MFMA tiles per wave along i (4 waves along i)
+    constexpr int KC = NH / 32;          // k chunks per pass
+    constexpr int NT = NH / 16;          // 16-row tiles of a basis
+    constexpr int PER = NH * NH / 2048;  // float4 loads per thread
+    __shared__ __attribute__((aligned(16))) _Float16 Xs[2][NH * LDX];
+    __shared__ float red[8];
+
+    const FieldGroup& g = a.g;
+    const long plane = blockIdx.x >> 2;
+    const int q = blockIdx.x & 3, r = q >> 1, c = q & 1;
+    const long fmp = plane >> 1;
+    const int part = (int)(plane & 1);
+    int f, mode;
+    plane_decode(g, fmp, f, mode);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int n = g.n;
+    const int hh[2] = {(n + 1) / 2, n / 2};
+    const size_t qoff = ((size_t)plane * 4 + q) * NH * NH;
+    const float* X = a.X + qoff;
+    float* Y = a.Y + qoff;
+
+    if (part == 1 && g.mode0 + mode == 0) {   // Im of mode 0 is not solved (int2d3d.f90:43)
+        for (int e = tid; e < NH * NH / 4; e += 512) ((f4v*)Y)[e] = f4v{0.f, 0.f, 0.f, 0.f};
+        return;
+    }
+    const double xl64 = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
+    const float xlam = (float)xl64;
+    const bool gauge = a.gauge >= 0 && q == 0 && fabs(xl64) <= 1.0e-10;
+
+    auto block_max = [&](float v) -> float {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+        if (lane == 0) red[wave] = v;
+        __syncthreads();
+        float mx = red[0];
+#pragma unroll
+        for (int w = 1; w < 8; ++w) mx = fmaxf(mx, red[w]);
+        return mx;
+    };
+    auto scale_of = [](float mx) -> float {   // power of two s with mx * s in [0.5, 1)
+        if (!(mx > 0.f) || !(mx < 3.0e38f)) return 1.f;
+        int e;
+        (void)frexpf(mx, &e);
+        return ldexpf(1.f, -e);
+    };
+    auto put4 = [&](int col, int row, f4v v, float s) {   // 4 consecutive rows of a column
+        h4v hi, lo;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float x = v[e] * s;
+            const _Float16 h = (_Float16)x;
+            hi[e] = h;
+            lo[e] = (_Float16)((x - (float)h) * 2048.f);
+        }
+        *(h4v*)&Xs[0][col * LDX + row] = hi;
+        *(h4v*)&Xs[1][col * LDX + row] = lo;
+    };
+
+    // ---- load X (column z', rows y'), scale and split
+    float sx;
+    {
+        f4v xv[PER];
+        float mx = 0.f;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            xv[u] = ((const f4v*)X)[tid + 512 * u];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) mx = fmaxf(mx, fabsf(xv[u][e]));
+        }
+        sx = scale_of(block_max(mx));
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int idx = (tid + 512 * u) * 4;
+            put4(idx / NH, idx % NH, xv[u], sx);
+        }
+        __syncthreads();
+    }
+
+    const int wj = wave & 1, wi = wave >> 1;   // 2 x 4 waves
+    const int j0 = wj * (NH / 2), i0 = wi * (NH / 4);
+    // the B fragments of a whole pass are loaded at once (2 terms x TI tiles x KC chunks),
+    // the next pass's issued before this pass's epilogue and its barriers
+    h8v bq[2][TI][KC];
+    auto load_b = [&](int pass) {
+        const int par = (pass == 0 || pass == 2) ? r : c;
+        const h8v* Bf = (const h8v*)(pass < 2 ? a.fwd[par] : a.bwd[par]);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int y = 0; y < TI; ++y)
+#pragma unroll
+                for (int kc = 0; kc < KC; ++kc)
+                    bq[t][y][kc] = Bf[((size_t)(t * NT + i0 / 16 + y) * KC + kc) * 64 + lane];
+    };
+    load_b(0);
+    for (int pass = 0; pass < 4; ++pass) {
+        const int par = (pass == 0 || pass == 2) ? r : c;
+        const float sb = pass < 2 ? a.sfwd[par] : a.sbwd[par];
+        f4v am[TJ][TI], ac[TJ][TI];
+#pragma unroll
+        for (int x = 0; x < TJ; ++x)
+#pragma unroll
+            for (int y = 0; y < TI; ++y) am[x][y] = ac[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+            h8v ah[TJ], al[TJ];
+#pragma unroll
+            for (int t = 0; t < TJ; ++t) {
+                const int j = j0 + t * 16 + l16;
+                const int k = kc * 32 + l4 * 8;
+                ah[t] = *(const h8v*)&Xs[0][j * LDX + k];
+                al[t] = *(const h8v*)&Xs[1][j * LDX + k];
+            }
+#pragma unroll
+            for (int y = 0; y < TI; ++y)
+#pragma unroll
+                for (int x = 0; x < TJ; ++x) {
+                    am[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[x], bq[0][y][kc], am[x][y], 0, 0, 0);
+                    ac[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[x], bq[1][y][kc], ac[x][y], 0, 0, 0);
+                    ac[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[x], bq[0][y][kc], ac[x][y], 0, 0, 0);
+                }
+        }
+        if (pass < 3) load_b(pass + 1);
+        // true values: (am + ac / 2048) / (sx sb), kept in am; D[j][i]: lane i + 16 (j/4), element j%4
+        const float inv = 1.f / (sx * sb);
+        float mx = 0.f;
+#pragma unroll
+        for (int x = 0; x < TJ; ++x)
+#pragma unroll
+            for (int y = 0; y < TI; ++y)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float t = (am[x][y][e] + ac[x][y][e] * (1.f / 2048.f)) * inv;
+                    if (pass == 1) {   // G = ... ./ (lam_r(j) + lam_c(i) - xlambda), solvediag.f90:61-66, :151-159
+                        const int j = j0 + x * 16 + l4 * 4 + e, i = i0 + y * 16 + l16;
+                        if (j < hh[r] && i < hh[c] && !(gauge && j == a.gauge && i == a.gauge))
+                            t = t / (a.lam[r * NH + j] + a.lam[c * NH + i] - xlam);
+                        else
+                            t = 0.f;
+                    }
+                    am[x][y][e] = t;
+                    mx = fmaxf(mx, fabsf(t));
+                }
+        if (pass == 3) {   // U(y' = j, z' = i) -> column i, rows j
+#pragma unroll
+            for (int x = 0; x < TJ; ++x)
+#pragma unroll
+                for (int y = 0; y < TI; ++y) {
+                    const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
+                    *(f4v*)&Y[(size_t)i * NH + j] = am[x][y];
+                }
+            break;
+        }
+        sx = scale_of(block_max(mx));   // the barrier inside: every wave is done reading Xs
+#pragma unroll
+        for (int x = 0; x < TJ; ++x)
+#pragma unroll
+            for (int y = 0; y < TI; ++y) {
+                const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
+                put4(i, j, am[x][y], sx);
+            }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_lpq(int NH, const LpqArgs& a, hipStream_t st)
+{
+    const long planes = (long)a.g.nf * a.g.nmodes * 2;
+    if (planes <= 0) return hipSuccess;
+    const dim3 grid((unsigned)(planes * 4)), block(512);
+    switch (NH) {
+        case 64: hipLaunchKernelGGL((k_lpq<64>), grid, block, 0, st, a); break;
+        case 128: hipLaunchKernelGGL((k_lpq<128>), grid, block, 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+float build_lpq_basis(const double* B, int ld, int NH, std::vector<unsigned short>& frag)
+{
+    // frag[((t NT + it) KC + kc) 64 + lane][e] = term t of B(it 16 + lane%16, kc 32 + (lane/16) 8 + e)
+    double mx = 0.0;
+    for (int k = 0; k < NH; ++k)
+        for (int i = 0; i < NH; ++i) mx = std::fmax(mx, std::fabs(B[i + (size_t)k * ld]));
+    int ex = 0;
+    if (mx > 0.0) (void)std::frexp(mx, &ex);
+    const float s = std::ldexp(1.f, -ex);
+    const int NT = NH / 16, KC = NH / 32;
+    frag.assign((size_t)2 * NT * KC * 64 * 8, 0);
+    for (int it = 0; it < NT; ++it)
+        for (int kc = 0; kc < KC; ++kc)
+            for (int lane = 0; lane < 64; ++lane)
+                for (int e = 0; e < 8; ++e) {
+                    const int i = it * 16 + (lane & 15), k = kc * 32 + (lane >> 4) * 8 + e;
+                    const float x = (float)B[i + (size_t)k * ld] * s;
+                    const _Float16 h = (_Float16)x;
+                    const _Float16 l = (_Float16)((x - (float)h) * 2048.f);
+                    unsigned short hb, lb;
+                    std::memcpy(&hb, &h, 2);
+                    std::memcpy(&lb, &l, 2);
+                    frag[((((size_t)0 * NT + it) * KC + kc) * 64 + lane) * 8 + e] = hb;
+                    frag[((((size_t)1 * NT + it) * KC + kc) * 64 + lane) * 8 + e] = lb;
+                }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Folded FP64 residual, one 64 x 64 block of one quarter of one complex plane
+// (re and im) per workgroup; the k_colloc scheme of resid.hip with the folded
+// blocks S_r (y term, B operand) and S_c (z term, A operand), K = NH.
+namespace {
+constexpr int RB = 64, RKT = 16, RLD = RKT + 1, kXcd = 8;
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* p, double v)
+{
+    atomicMax(p, (unsigned long long)__builtin_bit_cast(unsigned long long, v));
+}
+}  // namespace
+
+__global__ __launch_bounds__(256, 2) void k_resid_fold(ResidFoldArgs a, long ncplx)
+{
+    const FieldGroup& g = a.g;
+    const int NH = a.NH, m = g.m, n = g.n;
+    const int bpd = NH / RB, tpp = 4 * bpd * bpd;   // blocks per complex plane
+    const long bid = blockIdx.x;
+    const long grp = bid / ((long)kXcd * tpp);
+    const long fmp = grp * kXcd + bid % kXcd;       // the blocks of one plane share an XCD
+    const int blk = (int)((bid / kXcd) % tpp);
+    if (fmp >= ncplx) return;
+    int f, mode;
+    plane_decode(g, fmp, f, mode);
+    const int q = blk / (bpd * bpd), r = q >> 1, c = q & 1;
+    const int b2 = blk % (bpd * bpd);
+    const int z0 = (b2 / bpd) * RB, y0 = (b2 % bpd) * RB;
+    const int hh[2] = {(n + 1) / 2, n / 2};
+
+    __shared__ double Uz[2][2][RB * RLD];   // [buf][part][y][k]: u(y0 + y, k0 + k)
+    __shared__ double Uy[2][2][RB * RLD];   // [buf][part][z][k]: u(k0 + k, z0 + z)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int wz = wave & 1, wy = wave >> 1;
+
+    size_t qbase[2];
+#pragma unroll
+    for (int part = 0; part < 2; ++part) qbase[part] = ((size_t)(2 * fmp + part) * 4 + q) * NH * NH;
+    // Uz loader: k = tid/16 (z' = k0 + k), y = (tid%16)*4..+3;  Uy loader: z = tid/4, k = (tid%4)*4..+3 (y')
+    const int uz_k = tid >> 4, uz_y = (tid & 15) * 4;
+    const int uy_z = tid >> 2, uy_k = (tid & 3) * 4;
+    double rz[2][4], ry[2][4];
+    auto load = [&](int ks) {
+        const int k0 = ks * RKT;
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+            // sums in FP64 of the FP32 sources
+            double sz[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0};
+            for (int t = 0; t < a.nsrc; ++t) {
+                const f4v vz = ((const f4v*)(a.src[t] + qbase[part] + (size_t)(k0 + uz_k) * NH + y0 + uz_y))[0];
+                const f4v vy = ((const f4v*)(a.src[t] + qbase[part] + (size_t)(z0 + uy_z) * NH + k0 + uy_k))[0];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    sz[e] += (double)vz[e];
+                    sy[e] += (double)vy[e];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                rz[part][e] = sz[e];
+                ry[part][e] = sy[e];
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int part = 0; part < 2; ++part)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                Uz[buf][part][(uz_y + e) * RLD + uz_k] = rz[part][e];
+                Uy[buf][part][uy_z * RLD + uy_k + e] = ry[part][e];
+            }
+    };
+    const d2v* Sr = (const d2v*)a.Sfrag[r];
+    const d2v* Sc = (const d2v*)a.Sfrag[c];
+    auto dfrag = [&](const d2v* D, int rowtile, int kp) -> d2v {
+        return D[((size_t)rowtile * (NH / 8) + kp) * 64 + lane];
+    };
+
+    d4v acc[2][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) acc[i][j][p] = d4v{0.0, 0.0, 0.0, 0.0};
+    const int zt0 = (z0 + wz * 32) / 16, yt0 = (y0 + wy * 32) / 16;
+    const int NK = NH / RKT;
+    d2v az[2][2], by[2][2], azn[2][2], byn[2][2];   // matrix fragments, this and the next stage
+    auto load_frags = [&](int ks, d2v (&za)[2][2], d2v (&yb)[2][2]) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                za[t][h] = dfrag(Sc, zt0 + t, ks * 2 + h);
+                yb[t][h] = dfrag(Sr, yt0 + t, ks * 2 + h);
+            }
+    };
+    load(0);
+    load_frags(0, az, by);
+    store(0);
+    __syncthreads();
+    for (int ks = 0; ks < NK; ++ks) {
+        const int buf = ks & 1;
+        if (ks + 1 < NK) {
+            load(ks + 1);
+            load_frags(ks + 1, azn, byn);
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int kl = kk * 4 + l4;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                double bu[2], au[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    bu[t] = Uz[buf][p][(wy * 32 + t * 16 + l16) * RLD + kl];   // B[k][y] = u(y, k)
+                    au[t] = Uy[buf][p][(wz * 32 + t * 16 + l16) * RLD + kl];   // A[z][k] = u(k, z)
+                }
+#pragma unroll
+                for (int zt = 0; zt < 2; ++zt)
+#pragma unroll
+                    for (int yt = 0; yt < 2; ++yt) {
+                        acc[zt][yt][p] = __builtin_amdgcn_mfma_f64_16x16x4f64(az[zt][kk >> 1][kk & 1], bu[yt],
+                                                                              acc[zt][yt][p], 0, 0, 0);
+                        acc[zt][yt][p] = __builtin_amdgcn_mfma_f64_16x16x4f64(au[zt], by[yt][kk >> 1][kk & 1],
+                                                                              acc[zt][yt][p], 0, 0, 0);
+                    }
+            }
+        }
+        if (ks + 1 < NK) {
+            store(buf ^ 1);
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    az[t][h] = azn[t][h];
+                    by[t][h] = byn[t][h];
+                }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: D[i = z][j = y] in lane j + 16 (i%4), element i/4
+    const double xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
+    const bool zim = (g.mode0 + mode) == 0;
+    const bool singular = a.neumann && fabs(xlam) <= 1.0e-10;
+    const d2v* F = g.in[f] + (size_t)mode * m * m;
+    const double sr = r ? -1.0 : 1.0, sc = c ? -1.0 : 1.0;
+    double rmx = 0.0, fmx = 0.0;
+#pragma unroll
+    for (int zt = 0; zt < 2; ++zt)
+#pragma unroll
+        for (int yt = 0; yt < 2; ++yt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int z = z0 + wz * 32 + zt * 16 + 4 * e + l4;
+                const int y = y0 + wy * 32 + yt * 16 + l16;
+                d2v rv = {0.0, 0.0};
+                if (y < hh[r] && z < hh[c]) {
+                    const int ya = y + 1, yb = n - y, za = z + 1, zb = n - z;
+                    const d2v fa = F[ya + (size_t)za * m], fb = F[yb + (size_t)za * m];
+                    const d2v fc = F[ya + (size_t)zb * m], fd = F[yb + (size_t)zb * m];
+                    const d2v fv = 0.25 * ((fa + sr * fb) + sc * (fc + sr * fd));
+                    d2v u;
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        double s = 0.0;
+                        for (int t = 0; t < a.nsrc; ++t) s += (double)a.src[t][qbase[p] + (size_t)z * NH + y];
+                        u[p] = s;
+                    }
+                    const d2v lu = {acc[zt][yt][0][e], acc[zt][yt][1][e]};
+                    rv = fv - (lu - xlam * u);
+                    if (zim) rv[1] = 0.0;
+                    if (!singular) rmx = fmax(rmx, fmax(fabs(rv[0]), fabs(rv[1])));
+                    fmx = fmax(fmx, fmax(fabs(fv[0]), zim ? 0.0 : fabs(fv[1])));
+                }
+#pragma unroll
+                for (int p = 0; p < 2; ++p) a.R[qbase[p] + (size_t)z * NH + y] = (float)rv[p];
+            }
+    if (a.stats) {
+#pragma unroll
+        for (int sh = 32; sh > 0; sh >>= 1) {
+            rmx = fmax(rmx, __shfl_xor(rmx, sh, 64));
+            fmx = fmax(fmx, __shfl_xor(fmx, sh, 64));
+        }
+        if (lane == 0) {
+            atomic_max_pos(a.stats + 2 * a.slot[f], rmx);
+            atomic_max_pos(a.stats + 2 * a.slot[f] + 1, fmx);
+        }
+    }
+}
+
+hipError_t launch_resid_fold(const ResidFoldArgs& a, hipStream_t st)
+{
+    if (a.NH % RB || a.nsrc < 1 || a.nsrc > kMaxFoldSources) return hipErrorInvalidValue;
+    const long ncplx = (long)a.g.nf * a.g.nmodes;
+    if (ncplx <= 0) return hipSuccess;
+    const int bpd = a.NH / RB, tpp = 4 * bpd * bpd;
+    const long groups = (ncplx + kXcd - 1) / kXcd;
+    hipLaunchKernelGGL(k_resid_fold, dim3((unsigned)(groups * kXcd * tpp)), dim3(256), 0, st, a, ncplx);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+constexpr int kAux = 1024;
+
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < kAux / 64; ++w) s += red[w];
+    return s;
+}
+
+__global__ __launch_bounds__(kAux) void k_nullproj_fold(NullFoldArgs a, GaugeList list)
+{
+    const int plane = list.plane[blockIdx.x];        // 2 fmp + part
+    const int NH = a.NH, tid = threadIdx.x;
+    float* Re = a.R + (size_t)plane * 4 * NH * NH;   // quarter ee of the plane
+    __shared__ double red[kAux / 64];
+    double s = 0.0;
+    for (int e = tid; e < NH * NH; e += kAux) {
+        const int z = e / NH, y = e % NH;
+        s += a.qf[y] * (double)Re[e] * a.qf[z];
+    }
+    s = block_sum(s, red);
+    double mx = 0.0;
+    for (int e = tid; e < NH * NH; e += kAux) {
+        const int z = e / NH, y = e % NH;
+        const float v = (float)((double)Re[e] - s * a.pf[y] * a.pf[z]);
+        Re[e] = v;
+        mx = fmax(mx, fabs((double)v));
+    }
+    for (int e = tid; e < 3 * NH * NH; e += kAux) mx = fmax(mx, fabs((double)Re[(size_t)NH * NH + e]));
+    if (a.stats) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+        if ((tid & 63) == 0) {
+            const int f = (int)((plane >> 1) % a.nf);
+            atomic_max_pos(a.stats + 2 * a.slot[f], mx);
+        }
+    }
+}
+
+hipError_t launch_nullproj_fold(const NullFoldArgs& a, const GaugeList& list, hipStream_t st)
+{
+    if (list.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nullproj_fold, dim3(list.count), dim3(kAux), 0, st, a, list);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kAux) void k_gauge_fix(GaugeFixArgs a, GaugeList list)
+{
+    const FieldGroup& g = a.g;
+    const int plane = list.plane[blockIdx.x];        // 2 fmp + part
+    const long fmp = plane >> 1;
+    const int part = plane & 1;
+    int f, mode;
+    plane_decode(g, fmp, f, mode);
+    const int m = g.m, n = g.n, io = a.a;
+    double* U = (double*)(g.out[f] + (size_t)mode * m * m) + part;   // stride 2 doubles
+    __shared__ double cs;
+    if (threadIdx.x == 0) {
+        const double pa = a.p[io];
+        cs = (a.wanted - U[2 * ((size_t)(io + 1) + (size_t)(io + 1) * m)]) / (pa * pa);
+    }
+    __syncthreads();
+    const double cc = cs;
+    for (int e = threadIdx.x; e < n * n; e += kAux) {
+        const int z = e / n, y = e % n;
+        U[2 * ((size_t)(y + 1) + (size_t)(z + 1) * m)] += cc * a.p[y] * a.p[z];
+    }
+}
+
+hipError_t launch_gauge_fix(const GaugeFixArgs& a, const GaugeList& list, hipStream_t st)
+{
+    if (list.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gauge_fix, dim3(list.count), dim3(kAux), 0, st, a, list);
+    return hipGetLastError();
+}
+
+}  // namespace duct

@@ -63,6 +63,10 @@ int fold_size(int n);
 // Classify P's columns by parity and build the folded P^-1 (fwd) and P (bwd)
 // fragments, each (2 NH) x NH, and the eigenvalues per parity.  Returns -1 if
 // the eigenvectors do not split into ceil(n/2) even and floor(n/2) odd ones.
+// The folded matrices themselves: Af = [P_e^-1; P_o^-1], Ab = [P_e; P_o], each
+// (2 NH) x NH column-major (rows of parity p start at p NH).
+int fold_matrices(const double* P, const double* Pinv, const double* lam, int n, int NH, double* Af, double* Ab,
+                  double* lamf, int* pos);
 // pos (may be null) receives, for every eigenvector k, its index within its
 // parity (even: 0..h_e-1) or -1-index for odd ones.
 int build_fold_bases(const double* P, const double* Pinv, const double* lam, int n, int NH,

@@ -32,6 +32,7 @@
 // (2 x 2 M-tiles), so the unfold needs no exchange between waves; the data
 // tile is staged in LDS per parity (2 x 16 k x NT, double buffered).  A
 // group of 64 virtual columns = 16 folded columns x (cp, part) 4 N-tiles.
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -384,8 +385,8 @@ int fold_size(int n)
     return -1;
 }
 
-int build_fold_bases(const double* P, const double* Pinv, const double* lam, int n, int NH,
-                     double* fwd, double* bwd, double* lamf, int* pos)
+int fold_matrices(const double* P, const double* Pinv, const double* lam, int n, int NH, double* Af, double* Ab,
+                  double* lamf, int* pos)
 {
     const int he = (n + 1) / 2, ho = n / 2;
     std::vector<int> ev, od;
@@ -404,7 +405,8 @@ int build_fold_bases(const double* P, const double* Pinv, const double* lam, int
         for (int r = 0; r < ho; ++r) pos[od[r]] = -1 - r;
     }
     const int R = 2 * NH;
-    std::vector<double> Af((size_t)R * NH, 0.0), Ab((size_t)R * NH, 0.0);   // column-major, lda = R
+    std::fill(Af, Af + (size_t)R * NH, 0.0);
+    std::fill(Ab, Ab + (size_t)R * NH, 0.0);
     for (int c = 0; c < 2; ++c) {
         const std::vector<int>& ks = c ? od : ev;
         const int h = c ? ho : he;
@@ -423,6 +425,15 @@ int build_fold_bases(const double* P, const double* Pinv, const double* lam, int
         }
         for (int r = h; r < NH; ++r) lamf[c * NH + r] = 0.0;
     }
+    return 0;
+}
+
+int build_fold_bases(const double* P, const double* Pinv, const double* lam, int n, int NH,
+                     double* fwd, double* bwd, double* lamf, int* pos)
+{
+    const int R = 2 * NH;
+    std::vector<double> Af((size_t)R * NH), Ab((size_t)R * NH);
+    if (fold_matrices(P, Pinv, lam, n, NH, Af.data(), Ab.data(), lamf, pos)) return -1;
     build_frag_rect(Af.data(), R, NH, R, fwd);
     build_frag_rect(Ab.data(), R, NH, R, bwd);
     return 0;

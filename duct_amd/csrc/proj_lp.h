@@ -12,6 +12,7 @@ enum LpScheme : int {
     LP_BF16X3 = 2,  // bf16, 2-term split, 3 products
     LP_BF16X6 = 3,  // bf16, 3-term split, 6 products
     LP_BF16X1 = 4,  // plain bf16
+    LP_FOLD16 = 5,  // even/odd folded, fused per-quarter solve, fp16 hi/lo split (3 products)
 };
 
 struct ProjLpArgs {

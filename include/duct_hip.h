@@ -120,7 +120,12 @@ int duct_solvediag_(double *U, const char *bound, const double *xlambda, size_t 
  * the Neumann gauge, edges and corners:
  *   u = S[scheme0](f);  niter times: r = f - L u,  u += S[scheme1](r).
  * scheme: 1 = FP32 MFMA, 2 = bf16 split in 2 terms (3 products), 3 = bf16
- * split in 3 terms (6 products), 4 = plain bf16.  niter in [0, 16].
+ * split in 3 terms (6 products), 4 = plain bf16, 5 = the even/odd folded
+ * solve: all four passes of a quarter fused in one workgroup, fp16 hi/lo
+ * split (3 products), FP32 intermediates, the FP64 residual of the
+ * centrosymmetric part of D2B in the folded basis, one extra correction of
+ * the Poisson modes with xlambda < 64 (scheme0 == scheme1 == 5, niter <= 2,
+ * m with 32 < ceil((m-2)/2) <= 128).  niter in [0, 16].
  * stats (host, may be NULL): (niter+1) x nfields x 2 doubles = max|r| after
  * each step (the last one after the final correction) and max|f|.           */
 int duct_helm3d_mixed_batch_dev_(const int *nfields, const duct_devptr_t *rhs,

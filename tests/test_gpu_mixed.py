@@ -53,7 +53,8 @@ def run_mixed(dev, fields, bounds, alams, **kw):
 SCHEMES = {
     33: [("bf16x3", "bf16x6", 1), ("bf16x3", "bf16x3", 2), ("f32", "f32", 1), ("bf16x6", "bf16x3", 1),
          ("bf16x6", "bf16x6", 1)],
-    129: [("bf16x3", "bf16x6", 2), ("bf16x3", "bf16x3", 2), ("f32", "f32", 1), ("bf16x6", "bf16x6", 1)],
+    129: [("bf16x3", "bf16x6", 2), ("bf16x3", "bf16x3", 2), ("f32", "f32", 1), ("bf16x6", "bf16x6", 1),
+          ("fold16", "fold16", 1)],
 }
 CASES = [(m, M, s0, s1, it) for m, M in ((33, 16), (129, 8)) for s0, s1, it in SCHEMES[m]]
 
@@ -149,3 +150,43 @@ def test_mixed_full_size_config2(dev, oracle_mod):
                 assert e <= TOL
     finally:
         oracle_mod.use_blas(False)
+
+
+def test_fold16_full_size_config2(dev, oracle_mod):
+    """257^2 x 256 modes x 4 fields, the folded fused solve with one refinement (scheme 5):
+    <= 1e-10 on sampled modes, the low Poisson modes included."""
+    from duct_amd import workloads
+    cfg = workloads.CONFIGS[2]
+    ensure(dev, cfg.m)
+    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(cfg.m))
+    fields = [workloads.rhs_field(cfg, f) for f in range(4)]
+    alams = [workloads.alambda_of(cfg, f) for f in range(4)]
+    got, st = run_mixed(dev, fields, workloads.BOUNDS, alams, scheme0="fold16", scheme1="fold16", niter=1)
+    print("fold16 residual history:", [" ".join("%.1e" % (r / fm) for r, fm in st[:, f]) for f in range(4)],
+          flush=True)
+    oracle_mod.use_blas(True)
+    try:
+        for f in range(4):
+            for k0 in (0, 1, 2, 7, 8, 100, 255):
+                ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
+                e = relerr(got[f][k0:k0 + 1], ref)
+                print("fold16 config 2 field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
+                assert e <= TOL
+    finally:
+        oracle_mod.use_blas(False)
+    assert not got[0][0].imag.any()
+
+
+def test_fold16_rejects_unsupported_sizes(dev):
+    import torch
+    from duct_amd._lib import DuctError
+    ensure(dev, 33)
+    x = torch.zeros((2, 33, 33), dtype=torch.complex128, device="cuda")
+    y = torch.zeros_like(x)
+    with pytest.raises(DuctError):
+        dev.helm3d_mixed_batch([x], [y], "d", [1000.0], scheme0="fold16", scheme1="fold16", niter=1)
+    ensure(dev, 129)
+    x = torch.zeros((2, 129, 129), dtype=torch.complex128, device="cuda")
+    y = torch.zeros_like(x)
+    with pytest.raises(DuctError):
+        dev.helm3d_mixed_batch([x], [y], "d", [1000.0], scheme0="fold16", scheme1="bf16x6", niter=1)
