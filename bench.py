@@ -12,8 +12,12 @@ Paths timed on the same inputs:
                ~1e-13 of the oracle), the Neumann field unfolded
   fp64_fold    every field folded, the Neumann k=0 gauge included (~5e-11 of the
                oracle on the Poisson planes, their own conditioning)
-  mixed        the refined form: low-precision passes (FP32 or split-bf16 MFMA),
-               FP64 residual, correction(s) accumulated into the FP64 solution
+  mixed        the refined form of --mixed (default bf16x6,bf16x6,1): low-precision
+               passes (FP32 or split-bf16 MFMA), FP64 residual, correction(s)
+               accumulated into the FP64 solution
+  mixed_fold   the refined form in the folded basis (scheme fold16): fused per-quarter
+               fp16 hi/lo solves, FP64 residual of the folded operator, one correction
+               (two for the low Poisson modes)
 The reported `value` is the fastest path whose result is within 1e-10 (relative,
 max norm, every field) of the oracle; every path's throughput, error and kernel
 times are in the line.
@@ -55,7 +59,8 @@ def parse():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--mixed", default="bf16x6,bf16x6,1",
                     help="scheme0,scheme1,niter of the refined solve")
-    ap.add_argument("--path", choices=["auto", "fp64", "fp64_fold_d", "fp64_fold", "mixed"], default="auto")
+    ap.add_argument("--path", choices=["auto", "fp64", "fp64_fold_d", "fp64_fold", "mixed", "mixed_fold"],
+                    default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -116,7 +121,7 @@ def main():
     alams = [workloads.alambda_of(cfg, f) for f in range(F)]
     bounds = workloads.BOUNDS[:F]
     rhs = [torch.from_numpy(x).cuda() for x in fields]
-    paths = ("fp64", "fp64_fold_d", "fp64_fold", "mixed")
+    paths = ("fp64", "fp64_fold_d", "fp64_fold", "mixed", "mixed_fold")
     sol = {p: [torch.zeros_like(r) for r in rhs] for p in paths}
 
     def fp64_step(path, fold):
@@ -131,18 +136,28 @@ def main():
         "fp64_fold": fp64_step("fp64_fold", 2),
         "mixed": lambda: duct_amd.helm3d_mixed_batch(rhs, sol["mixed"], bounds, alams, alp=cfg.alp, ib=0,
                                                      scheme0=s0, scheme1=s1, niter=niter),
+        "mixed_fold": lambda: duct_amd.helm3d_mixed_batch(rhs, sol["mixed_fold"], bounds, alams, alp=cfg.alp,
+                                                          ib=0, scheme0="fold16", scheme1="fold16", niter=1),
     }
     he, ho = (n + 1) // 2, n // 2
     planes = F * (2 * M - 1)                      # Im at k=0 not solved
     nprod = {"f32": 1, "bf16x1": 1, "bf16x3": 3, "bf16x6": 6}
 
-    def flop_per_step(kind):
-        """Algorithmic FLOP of one step of a kernel class."""
+    def flop_per_step(kind, path):
+        """Algorithmic FLOP of one step of a kernel class (the extra correction of the low
+        Poisson modes in mixed_fold is not counted)."""
         g = 2.0 * n ** 3 * planes                  # one projection pass, one product
+        gf = 2.0 * (he * he + ho * ho) * n * planes  # one folded pass, one product
         if kind.startswith("proj_"):
             return g
-        if kind.startswith("fold_"):               # the folded pass: 2 (h_e^2 + h_o^2) n per plane
-            return 2.0 * (he * he + ho * ho) * n * planes
+        if kind.startswith("fold_"):
+            return gf
+        if path == "mixed_fold":
+            if kind == "lp_p2p":                   # k_lpq: 4 passes x 3 products, two solves
+                return 2 * 4 * 3 * gf
+            if kind == "resid":                    # folded residual: two products
+                return 2.0 * gf
+            return 0.0                             # fold / unfold: bandwidth-bound
         if kind.startswith("lp_"):
             return g * (nprod[s0] + niter * nprod[s1])
         if kind == "resid":
@@ -194,8 +209,9 @@ def main():
     dom = max(groups, key=lambda g: sum(ks[k]["ms_per_step"] for k in groups[g]))
     dom_ms = sum(ks[k]["ms_per_step"] for k in groups[dom])
     dom_launches = sum(ks[k]["launches_per_step"] for k in groups[dom])
-    dom_flop = sum(flop_per_step(k) for k in groups[dom])
-    unit_peak = {"proj": "f64", "fold": "f64", "resid": "f64", "lp": "f32" if s0 == "f32" else "bf16"}.get(dom, "f64")
+    dom_flop = sum(flop_per_step(k, chosen) for k in groups[dom])
+    lp_peak = "bf16" if chosen == "mixed_fold" or s0 != "f32" else "f32"
+    unit_peak = {"proj": "f64", "fold": "f64", "resid": "f64", "lp": lp_peak}.get(dom, "f64")
     avg_launch_s = dom_ms / 1e3 / dom_launches if dom_launches else float("nan")
     flop_per_launch = dom_flop / dom_launches if dom_launches else 0.0
     achieved = flop_per_launch / avg_launch_s if dom_launches else 0.0
@@ -212,7 +228,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if chosen.startswith("fp64") else "%s+%s refined, f64 residual" % (s0, s1),
+        "dtype": "f64" if chosen.startswith("fp64") else
+                 ("fp16 hi/lo folded, f64 residual" if chosen == "mixed_fold" else
+                  "%s+%s refined, f64 residual" % (s0, s1)),
         "data": "synthetic (seeded Chebyshev-series right-hand sides, inputs resident in HBM)",
         "config": {"workload": "cross-plane solve stage: %d^2 CGL x %d modes x %d fields (u,v,w Dirichlet "
                                "alambda=1000; p Neumann alambda=0)" % (m, M, F),
@@ -226,8 +244,9 @@ def main():
                                 "resid": "k_resid (FP64 MFMA residual)"}.get(dom, dom),
                      "flop_per_launch": flop_per_launch, "avg_launch_us": avg_launch_s * 1e6},
         "paths": res,
-        "mixed_vs_fp64": (res["mixed"]["mode_solves_per_s"] / res["fp64"]["mode_solves_per_s"]
-                          if "mixed" in res and "fp64" in res else None),
+        "mixed_vs_fp64": (max(res[p]["mode_solves_per_s"] for p in ("mixed", "mixed_fold") if p in res)
+                          / res["fp64"]["mode_solves_per_s"]
+                          if "fp64" in res and ("mixed" in res or "mixed_fold" in res) else None),
         "fp64_mfma_frac": None,
         "cpu_baseline": cpu,
     }

@@ -35,6 +35,9 @@ SIGNATURES = {
     "duct_colloc_deriv_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _pd, _pd, _sz]),
     "duct_derivx_dev_": (_i, [_psz, _psz, _pi, _pi, _pi, _pd, _pd, _pd]),
     "duct_advection_dev_": (_i, [_psz, _psz, _psz, _psz, _psz, _psz, _pi, _pi, _pi, _pd, _pd]),
+    "duct_transyz_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _sz]),
+    "duct_fourx_dev_": (_i, [_psz, _psz, _pi, _pi, _pi, _pi, _pi]),
+    "duct_derivyz_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _pd, _pc, _pd, _sz, _sz]),
     "duct_transpose_dev_": (_i, [_pi, _pi, _pi, _pd, _psz, _psz]),
     "duct_cxyz2yzx_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),
     "duct_cyzx2xyz_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),

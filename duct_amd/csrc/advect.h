@@ -18,6 +18,27 @@ struct XprodArgs {
 };
 // M % 16 == 0, M <= 64, X % 16 == 0, X <= 192.
 hipError_t launch_xprod(const XprodArgs& a, hipStream_t st);
+
+struct FourxArgs {
+    d2v* fou;              // fou(0:M-1, lines) complex, mode fastest
+    double* phys;          // phys(X+2, lines) real, x fastest
+    const double* Einv;    // as XprodArgs
+    const double* Fwd;
+    long lines;            // my * nz
+    int M, X, inverse;     // inverse: fou -> phys (fourx iopt >= 0)
+};
+hipError_t launch_fourx(const FourxArgs& a, hipStream_t st);
+
+struct DerivyzArgs {
+    const d2v* u;          // Chebyshev coefficients, complex(8) (m, m, nmodes)
+    d2v* du;               // result (= or +=)
+    d2v* aux;              // scratch of the same size (may be du when not accumulating)
+    int m, nmodes, dirz, ider, accumulate;
+    double scal;           // scal / aspect^ider for z
+};
+hipError_t launch_derivyz(const DerivyzArgs& a, hipStream_t st);
+// Chebyshev transform matrices (column-major m x m): fwd = phys -> coefficients, inv = the inverse.
+void build_cheb_transforms(int m, std::vector<double>& fwd, std::vector<double>& inv);
 void build_xdft(int M, int X, std::vector<double>& einv, std::vector<double>& fwd);
 
 }  // namespace duct

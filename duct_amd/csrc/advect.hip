@@ -155,6 +155,138 @@ static void frag_b(const std::vector<double>& src, int J, int K, std::vector<dou
             }
 }
 
+// ---------------------------------------------------------------------------
+// fourx (fourier.F:1-115) on the reference's z-cut layouts, 16 (y, z) lines per
+// workgroup, the same dense x-DFTs on the matrix cores:
+//   fou(0:M-1, my, nz) complex, mode fastest;  phys(mgalx+2, my, nz) real, x fastest
+//   (the two padding values of a physical line are written as zero).
+__global__ __launch_bounds__(256, 2) void k_fourx(FourxArgs a)
+{
+    const int M = a.M, Q = 2 * M, X = a.X, XP = a.X + 2;
+    const long lines = a.lines, L0 = (long)blockIdx.x * kLines;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    __shared__ double smem[kSmem];
+    if (a.inverse) {
+        for (int e = tid; e < M * kLines; e += 256) {          // coef[q][line], a line's modes contiguous
+            const int l = e / M, k = e % M;
+            d2v c = {0.0, 0.0};
+            if (L0 + l < lines) c = a.fou[(L0 + l) * M + k];
+            smem[(size_t)k * kLines + l] = c[0];
+            smem[(size_t)(M + k) * kLines + l] = c[1];
+        }
+        __syncthreads();
+        const int XT = X / 16;
+        for (int xt = wave; xt < XT; xt += 4) {
+            d4v acc = {0.0, 0.0, 0.0, 0.0};
+            for (int ks = 0; ks < Q / 4; ++ks)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(smem[(size_t)(ks * 4 + l4) * kLines + l16],
+                                                          a.Einv[((size_t)xt * (Q / 4) + ks) * 64 + lane], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long L = L0 + 4 * r + l4;
+                if (L < lines) a.phys[L * XP + xt * 16 + l16] = acc[r];
+            }
+        }
+        for (int l = tid; l < kLines * 2; l += 256) {            // the padding of each line
+            const long L = L0 + l / 2;
+            if (L < lines) a.phys[L * XP + X + (l & 1)] = 0.0;
+        }
+    } else {
+        for (int e = tid; e < X * kLines; e += 256) {           // p[x][line], a line's x contiguous
+            const int l = e / X, x = e % X;
+            smem[(size_t)x * kLdp + l] = (L0 + l < lines) ? a.phys[(L0 + l) * XP + x] : 0.0;
+        }
+        __syncthreads();
+        const int QT2 = M / 16;
+        for (int t = wave; t < QT2; t += 4) {
+            d4v ore = {0.0, 0.0, 0.0, 0.0}, oim = ore;
+            for (int ks = 0; ks < X / 4; ++ks) {
+                const double av = smem[(size_t)(ks * 4 + l4) * kLdp + l16];
+                ore = __builtin_amdgcn_mfma_f64_16x16x4f64(av, a.Fwd[((size_t)t * (X / 4) + ks) * 64 + lane], ore, 0, 0, 0);
+                oim = __builtin_amdgcn_mfma_f64_16x16x4f64(av, a.Fwd[((size_t)(t + QT2) * (X / 4) + ks) * 64 + lane],
+                                                          oim, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long L = L0 + 4 * r + l4;
+                if (L < lines) a.fou[L * M + t * 16 + l16] = d2v{ore[r], oim[r]};
+            }
+        }
+    }
+}
+
+hipError_t launch_fourx(const FourxArgs& a, hipStream_t st)
+{
+    if (a.M % 16 || a.M > kXprodMaxM || a.X % 16 || a.X > kXprodMaxX || a.lines <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fourx, dim3((unsigned)((a.lines + kLines - 1) / kLines)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// derivyz (chebyshev.f90:1-295): first or second derivative of Chebyshev
+// coefficients along y or z, one line per thread.  From the top coefficient
+// down, c_k = c_{k+2} + 2 k u_k, du_k = -c_{k+1}, du_0 = -c_1 / 2 (the minus
+// sign: the coefficients belong to cos(pi j / N) = -x_j).  The second
+// derivative repeats the recurrence on the first (through aux).
+__device__ __forceinline__ void cheb_recurrence(const d2v* in, d2v* out, long stride, int m)
+{
+    d2v aari = {0.0, 0.0}, bri = aari, tri = aari;
+    for (int kk = m - 1; kk >= 1; --kk) {
+        const d2v cri = bri + (2.0 * (double)kk) * in[kk * stride];
+        out[kk * stride] = -tri;
+        bri = aari;
+        tri = cri;
+        aari = cri;
+    }
+    out[0] = -0.5 * tri;
+}
+
+__global__ __launch_bounds__(256) void k_derivyz(DerivyzArgs a)
+{
+    const long line = (long)blockIdx.x * 256 + threadIdx.x;
+    const int m = a.m;
+    if (line >= (long)a.nmodes * m) return;
+    const long mode = line / m, t = line % m;
+    const size_t base = (size_t)mode * m * m + (a.dirz ? (size_t)t : (size_t)t * m);
+    const long stride = a.dirz ? m : 1;
+    const d2v* u = a.u + base;
+    d2v* w = a.aux + base;
+    cheb_recurrence(u, w, stride, m);
+    if (a.ider == 2) cheb_recurrence(w, w, stride, m);   // in place: each entry is read before it is written
+    d2v* du = a.du + base;
+    for (int k = 0; k < m; ++k) {
+        const d2v v = a.scal * w[k * stride];
+        du[k * stride] = a.accumulate ? du[k * stride] + v : v;
+    }
+}
+
+hipError_t launch_derivyz(const DerivyzArgs& a, hipStream_t st)
+{
+    const long lines = (long)a.nmodes * a.m;
+    if (lines <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_derivyz, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+void build_cheb_transforms(int m, std::vector<double>& fwd, std::vector<double>& inv)
+{
+    // tcheb_fftw (tchebw.f:62-150, nnyc = 0): c_k = s_k sum_j w_j cos(pi j k / N) u_j with
+    // w_j = 1 at the ends, 2 inside, s_k = 1/N inside, 1/(2N) at the ends; u_j = sum_k c_k cos(pi j k / N).
+    const int N = m - 1;
+    fwd.assign((size_t)m * m, 0.0);
+    inv.assign((size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j)
+        for (int k = 0; k < m; ++k) {
+            const long ph = ((long)j * k) % (2L * N);            // exact reduction of the angle
+            const double c = std::cos(3.14159265358979323846 * (double)ph / (double)N);
+            const double w = (j == 0 || j == N) ? 1.0 : 2.0;
+            const double sk = (k == 0 || k == N) ? 0.5 / N : 1.0 / N;
+            fwd[k + (size_t)j * m] = sk * w * c;                 // column-major: row k, column j
+            inv[j + (size_t)k * m] = c;                          // row j, column k
+        }
+}
+
 void build_xdft(int M, int X, std::vector<double>& einv, std::vector<double>& fwd)
 {
     const int Q = 2 * M;

@@ -86,6 +86,7 @@ struct Lib {
     size_t rfield_bytes = 0;
     unsigned long long* stats = nullptr;  // residual maxima of the mixed solve
     double* dcfrag[2] = {nullptr, nullptr}; // DC and DC^T, FP64 fragment layout (derivative products)
+    double* tfrag[2] = {nullptr, nullptr};  // Chebyshev transforms phys->cheb, cheb->phys (transyz)
     int dc_m = 0, dc_np = 0;
     int fold = 0;                         // 1: Dirichlet groups use the folded passes, 2: Neumann too
     double* xdft[2] = {nullptr, nullptr}; // advection x-DFTs (inverse, forward), fragment layout
@@ -882,6 +883,8 @@ int duct_shutdown_(void)
     free_ptr(L.stats);
     free_ptr(L.dcfrag[0]);
     free_ptr(L.dcfrag[1]);
+    free_ptr(L.tfrag[0]);
+    free_ptr(L.tfrag[1]);
     L.dc_m = L.dc_np = 0;
     free_ptr(L.xdft[0]);
     free_ptr(L.xdft[1]);
@@ -996,6 +999,14 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
             for (int i = 0; i < m; ++i) T[i + (size_t)j * m] = DC[j + (size_t)i * m];
         build_frag(T.data(), m, m, MP, f.data());
         if ((st = up(L.dcfrag[1], f.data(), f.size()))) return st;
+        std::vector<double> tf, ti;
+        build_cheb_transforms(m, tf, ti);
+        free_ptr(L.tfrag[0]);
+        free_ptr(L.tfrag[1]);
+        build_frag(tf.data(), m, m, MP, f.data());
+        if ((st = up(L.tfrag[0], f.data(), f.size()))) return st;
+        build_frag(ti.data(), m, m, MP, f.data());
+        if ((st = up(L.tfrag[1], f.data(), f.size()))) return st;
         L.dc_m = m;
         L.dc_np = MP;
     }
@@ -1432,6 +1443,50 @@ int duct_helm3d_adjoint_batch_dev_(const int* nfields, const duct_devptr_t* solb
     return OK;
 }
 
+// The advection x-DFT tables for (M, X), rebuilt when either changes.
+int ensure_xdft(int M, int X)
+{
+    if (L.xdft_M == M && L.xdft_X == X) return OK;
+    std::vector<double> ei, fw;
+    build_xdft(M, X, ei, fw);
+    HIPCHK(hipStreamSynchronize(L.stream));
+    free_ptr(L.xdft[0]);
+    free_ptr(L.xdft[1]);
+    L.xdft_M = L.xdft_X = 0;
+    HIPCHK(hipMalloc((void**)&L.xdft[0], ei.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(L.xdft[0], ei.data(), ei.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void**)&L.xdft[1], fw.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(L.xdft[1], fw.data(), fw.size() * sizeof(double), hipMemcpyHostToDevice));
+    L.xdft_M = M;
+    L.xdft_X = X;
+    return OK;
+}
+
+int duct_fourx_dev_(const duct_devptr_t* fou, const duct_devptr_t* phys, const int* my, const int* nz,
+                    const int* nmodes, const int* mgalx, const int* iopt)
+{
+    if (!fou || !phys || !my || !nz || !nmodes || !mgalx || !iopt) return fail(ERR_ARG, "null argument to duct_fourx_dev_");
+    if (!*fou || !*phys) return fail(ERR_ARG, "null device pointer");
+    if (*my < 1 || *nz < 1) return fail(ERR_ARG, "my=%d, nz=%d", *my, *nz);
+    const int M = *nmodes, X = *mgalx;
+    if (M % 16 || M > kXprodMaxM || X % 16 || X > kXprodMaxX || 2 * M > X)
+        return fail(ERR_UNSUPPORTED, "fourx needs nmodes %% 16 == 0, nmodes <= %d, mgalx %% 16 == 0, "
+                    "2 nmodes <= mgalx <= %d (got %d, %d)", kXprodMaxM, kXprodMaxX, M, X);
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = ensure_xdft(M, X))) return st;
+    FourxArgs a{};
+    a.fou = (d2v*)*fou;
+    a.phys = (double*)*phys;
+    a.Einv = L.xdft[0];
+    a.Fwd = L.xdft[1];
+    a.lines = (long)*my * *nz;
+    a.M = M; a.X = X; a.inverse = *iopt >= 0;
+    Timed t(PK_XPROD);
+    HIPCHK(launch_fourx(a, L.stream));
+    return OK;
+}
+
 int duct_advection_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const duct_devptr_t* w,
                         const duct_devptr_t* adv1, const duct_devptr_t* adv2, const duct_devptr_t* adv3,
                         const int* m, const int* nmodes, const int* mgalx, const double* alp, const double* aspect)
@@ -1454,20 +1509,7 @@ int duct_advection_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const du
     if (!(*aspect > 0.0)) return fail(ERR_ARG, "aspect must be positive");
     if (!L.dcfrag[0] || L.dc_m != mm)
         return fail(ERR_NOT_SETUP, "no collocation matrix for m=%d (duct_setup_ builds it)", mm);
-    if (L.xdft_M != M || L.xdft_X != X) {
-        std::vector<double> ei, fw;
-        build_xdft(M, X, ei, fw);
-        HIPCHK(hipStreamSynchronize(L.stream));
-        free_ptr(L.xdft[0]);
-        free_ptr(L.xdft[1]);
-        L.xdft_M = L.xdft_X = 0;
-        HIPCHK(hipMalloc((void**)&L.xdft[0], ei.size() * sizeof(double)));
-        HIPCHK(hipMemcpy(L.xdft[0], ei.data(), ei.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMalloc((void**)&L.xdft[1], fw.size() * sizeof(double)));
-        HIPCHK(hipMemcpy(L.xdft[1], fw.data(), fw.size() * sizeof(double), hipMemcpyHostToDevice));
-        L.xdft_M = M;
-        L.xdft_X = X;
-    }
+    if ((st = ensure_xdft(M, X))) return st;
     const size_t field = (size_t)M * mm * mm;
     if ((st = ensure_buf(L.prods, L.prods_bytes, 6 * field * sizeof(d2v)))) return st;
     XprodArgs xa{};
@@ -1507,6 +1549,67 @@ int duct_advection_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const du
             HIPCHK(launch_colloc(T.dir == 'y' ? CL_DY : CL_DZ, a, L.stream));
         }
     }
+    return OK;
+}
+
+int duct_transyz_dev_(const duct_devptr_t* in, const duct_devptr_t* out, const int* m, const int* nmodes,
+                      const char* cdir, const int* iopt, size_t cdir_len)
+{
+    (void)cdir_len;
+    if (!in || !out || !m || !nmodes || !cdir || !iopt) return fail(ERR_ARG, "null argument to duct_transyz_dev_");
+    if (*cdir != 'y' && *cdir != 'z') return fail(ERR_ARG, "err in transyz: cdir '%c'", *cdir);
+    if (!*in || !*out) return fail(ERR_ARG, "null device pointer");
+    if (*in == *out) return fail(ERR_ARG, "in and out must not alias");
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    if (!L.tfrag[0] || L.dc_m != *m)
+        return fail(ERR_NOT_SETUP, "no Chebyshev transforms for m=%d (duct_setup_ builds them)", *m);
+    CollocArgs a{};
+    a.g.nf = 1; a.g.nmodes = *nmodes; a.g.m = *m; a.g.n = *m - 2;
+    a.g.in[0] = (const d2v*)*in;
+    a.g.out[0] = (d2v*)*out;
+    a.Dfrag = L.tfrag[*iopt < 0 ? 0 : 1];
+    a.NP = L.dc_np;
+    a.scal = 1.0;
+    a.beta = 0.0;
+    Timed t(PK_DERIV);
+    HIPCHK(launch_colloc(*cdir == 'y' ? CL_DY : CL_DZ, a, L.stream));
+    return OK;
+}
+
+int duct_derivyz_dev_(const duct_devptr_t* u, const duct_devptr_t* du, const int* m, const int* nmodes,
+                      const char* cdir, const int* ider, const double* scal, const char* what, const double* aspect,
+                      size_t cdir_len, size_t what_len)
+{
+    (void)cdir_len;
+    (void)what_len;
+    if (!u || !du || !m || !nmodes || !cdir || !ider || !scal || !what || !aspect)
+        return fail(ERR_ARG, "null argument to duct_derivyz_dev_");
+    if (*cdir != 'y' && *cdir != 'z') return fail(ERR_ARG, "cdir '%c' is neither 'y' nor 'z'", *cdir);
+    if (*ider != 1 && *ider != 2) return fail(ERR_ARG, "this order of deriv not implemented: %d", *ider);
+    if (!*u || !*du) return fail(ERR_ARG, "null device pointer");
+    if (*u == *du) return fail(ERR_ARG, "u and du can not have the same address");
+    if (!(*aspect > 0.0)) return fail(ERR_ARG, "aspect must be positive");
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    DerivyzArgs a{};
+    a.u = (const d2v*)*u;
+    a.du = (d2v*)*du;
+    a.m = *m; a.nmodes = *nmodes; a.dirz = *cdir == 'z'; a.ider = *ider;
+    a.accumulate = *what != 'n';
+    a.scal = *scal;
+    if (a.dirz) a.scal = *ider == 1 ? *scal / *aspect : *scal / (*aspect * *aspect);
+    if (a.accumulate) {
+        const size_t bytes = (size_t)*nmodes * *m * *m * sizeof(d2v);
+        if ((st = ensure_buf(L.rfield, L.rfield_bytes, bytes))) return st;
+        a.aux = L.rfield;
+    } else {
+        a.aux = a.du;
+    }
+    Timed t(PK_DERIV);
+    HIPCHK(launch_derivyz(a, L.stream));
     return OK;
 }
 

@@ -11,7 +11,7 @@ module duct_hip
   public :: duct_init, duct_shutdown, duct_setup, duct_helm3d, duct_helm3d_dev, &
             duct_helm3d_batch_dev, duct_int2d3d, duct_solvediag, duct_check, duct_set_fold, &
             duct_helm3d_mixed_batch_dev, duct_helm3d_adjoint_batch_dev, duct_colloc_deriv_dev, &
-            duct_derivx_dev, duct_advection_dev
+            duct_derivx_dev, duct_advection_dev, duct_transyz_dev, duct_derivyz_dev, duct_fourx_dev
 
   interface
     integer(c_int) function duct_init(device) bind(C, name="duct_init_")
@@ -141,6 +141,33 @@ module duct_hip
       integer(c_size_t), intent(in) :: in, out
       integer(c_int), intent(in) :: m, nmodes, ib
       real(c_double), intent(in) :: alp, scal, beta
+    end function
+
+    ! fourx (fourier.F:1-115) on the z-cut layouts
+    integer(c_int) function duct_fourx_dev(fou, phys, my, nz, nmodes, mgalx, iopt) bind(C, name="duct_fourx_dev_")
+      import :: c_int, c_size_t
+      integer(c_size_t), intent(in) :: fou, phys
+      integer(c_int), intent(in) :: my, nz, nmodes, mgalx, iopt
+    end function
+
+    ! transyz (chebyshev.f90:458-540) and derivyz (:1-295) on device fields
+    integer(c_int) function duct_transyz_dev(in, out, m, nmodes, cdir, iopt, cdir_len) &
+        bind(C, name="duct_transyz_dev_")
+      import :: c_int, c_size_t, c_char
+      integer(c_size_t), intent(in) :: in, out
+      integer(c_int), intent(in) :: m, nmodes, iopt
+      character(kind=c_char), intent(in) :: cdir
+      integer(c_size_t), value :: cdir_len
+    end function
+
+    integer(c_int) function duct_derivyz_dev(u, du, m, nmodes, cdir, ider, scal, what, aspect, cdir_len, &
+        what_len) bind(C, name="duct_derivyz_dev_")
+      import :: c_int, c_double, c_size_t, c_char
+      integer(c_size_t), intent(in) :: u, du
+      integer(c_int), intent(in) :: m, nmodes, ider
+      character(kind=c_char), intent(in) :: cdir, what
+      real(c_double), intent(in) :: scal, aspect
+      integer(c_size_t), value :: cdir_len, what_len
     end function
 
     ! advection3d (operators.F:9-276) on device fields

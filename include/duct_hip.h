@@ -157,6 +157,33 @@ int duct_colloc_deriv_dev_(const duct_devptr_t *in, const duct_devptr_t *out, co
                            const int *nmodes, const char *cdir, const int *trans,
                            const double *scal, const double *beta, size_t cdir_len);
 
+/* fourx (src/fourier.F:1-115) on the reference's z-cut layouts:
+ * fou(0:nmodes-1, my, nz) complex(8) and phys(mgalx+2, my, nz) real(8).
+ * iopt >= 0: fou -> phys, modes above nmodes-1 zero (fill), FFTW c2r sum
+ * (Im of mode 0 ignored), the two padding values written as 0; iopt < 0:
+ * phys -> fou, the first nmodes modes of the r2c sum times 1/mgalx
+ * (pack_fftw).  Dense x-DFTs on the FP64 matrix cores; nmodes % 16 == 0,
+ * nmodes <= 64, mgalx % 16 == 0, 2 nmodes <= mgalx <= 192.                  */
+int duct_fourx_dev_(const duct_devptr_t *fou, const duct_devptr_t *phys, const int *my, const int *nz,
+                    const int *nmodes, const int *mgalx, const int *iopt);
+
+/* Chebyshev transform of complex(8) fields (m, m, nmodes) along y or z
+ * (transyz, src/chebyshev.f90:458-540, through tcheb_fftw, src/tchebw.f:62-150,
+ * REDFT00 with the reference's end scalings and no dealiasing): iopt < 0
+ * physical -> Chebyshev coefficients of cos(pi j/N), iopt >= 0 the inverse.
+ * Applied as dense m x m products on the FP64 matrix cores; out-of-place
+ * (in and out must differ; the reference transforms in place).           */
+int duct_transyz_dev_(const duct_devptr_t *in, const duct_devptr_t *out, const int *m, const int *nmodes,
+                      const char *cdir, const int *iopt, size_t cdir_len);
+
+/* derivyz (src/chebyshev.f90:1-295) on Chebyshev coefficients: du = scal *
+ * d^ider u / dy^ider (what 'n') or du += (any other what), z derivatives
+ * divided by aspect^ider; ider 1 or 2; u and du must differ.  The reference's
+ * backward coefficient recurrence, one line per thread.                     */
+int duct_derivyz_dev_(const duct_devptr_t *u, const duct_devptr_t *du, const int *m, const int *nmodes,
+                      const char *cdir, const int *ider, const double *scal, const char *what,
+                      const double *aspect, size_t cdir_len, size_t what_len);
+
 /* Fourier x-derivative: out = beta*out + scal * (i*alp*k) * in for modes
  * k = ib .. ib+nmodes-1 (derivx, src/fourier.F:117-188, ider = 1).          */
 int duct_derivx_dev_(const duct_devptr_t *in, const duct_devptr_t *out, const int *m,

@@ -146,3 +146,72 @@ def test_config3_step(dev, oracle_mod):
     rel = np.abs(div[:, 1:-1, 1:-1]).max() / np.abs(dstar[:, 1:-1, 1:-1]).max()
     print("config 3 interior divergence after projection / before: %.2e" % rel, flush=True)
     assert rel <= 1e-9
+
+
+@pytest.mark.parametrize("m", [33, 129])
+def test_transyz_and_derivyz_match_oracle(dev, m):
+    """duct_transyz_dev_ (dense transforms) and duct_derivyz_dev_ (the recurrence) against the
+    oracle's REDFT00 transforms and recurrence; transform + derivative + inverse against DC."""
+    import torch
+    from duct_amd import step, basis
+    from oracle import spectral as S
+    ensure(dev, m)
+    M = 3
+    rng = np.random.default_rng(m)
+    f = rng.standard_normal((M, m, m)) + 1j * rng.standard_normal((M, m, m))
+    for cdir in ("y", "z"):
+        for iopt in (-1, 1):
+            out = torch.zeros_like(cuda(f))
+            step.transyz(cuda(f), out, cdir, iopt)
+            torch.cuda.synchronize()
+            ref = S.transyz(f, cdir, iopt)
+            e = relerr(out.cpu().numpy(), ref)
+            print("m=%d transyz %s %+d rel.err %.2e" % (m, cdir, iopt, e), flush=True)
+            assert e <= 1e-13
+        for ider in (1, 2):
+            du = torch.zeros_like(cuda(f))
+            step.derivyz(cuda(f), du, cdir, ider=ider, scal=0.5, aspect=1.5)
+            g0 = rng.standard_normal((M, m, m)) + 1j * rng.standard_normal((M, m, m))
+            acc = cuda(g0)
+            step.derivyz(cuda(f), acc, cdir, ider=ider, scal=0.5, what="+", aspect=1.5)
+            torch.cuda.synchronize()
+            ref = S.derivyz(f, cdir, ider, scal=0.5, aspect=1.5)
+            e = relerr(du.cpu().numpy(), ref)
+            e2 = relerr(acc.cpu().numpy(), ref + g0)
+            print("m=%d derivyz %s ider=%d rel.err %.2e (accumulate %.2e)" % (m, cdir, ider, e, e2), flush=True)
+            assert e <= 1e-13 and e2 <= 1e-13
+    # the reference's route of a physical derivative equals the collocation product
+    c = torch.zeros_like(cuda(f))
+    d = torch.zeros_like(c)
+    p = torch.zeros_like(c)
+    step.transyz(cuda(f), c, "y", -1)
+    step.derivyz(c, d, "y", 1)
+    step.transyz(d, p, "y", 1)
+    torch.cuda.synchronize()
+    _, DC = basis.collocation_matrix(m)
+    want = f @ DC.T
+    assert np.max(np.abs(p.cpu().numpy() - want)) <= 1e-12 * np.abs(DC).sum(1).max() * np.abs(f).max()
+
+
+@pytest.mark.parametrize("my,nz,M,mgalx", [(9, 5, 16, 48), (129, 7, 64, 192)])
+def test_fourx_matches_oracle(dev, my, nz, M, mgalx):
+    """duct_fourx_dev_ against numpy's FFT restatement of fourx (fill, c2r, r2c, pack_fftw)."""
+    import torch
+    from duct_amd import step
+    from oracle import spectral as S
+    rng = np.random.default_rng(my + M)
+    c = rng.standard_normal((M, nz, my)) + 1j * rng.standard_normal((M, nz, my))   # [mode, z, y]
+    fou = cuda(np.ascontiguousarray(np.moveaxis(c, 0, -1)))                         # [z, y, mode]
+    phys = torch.full((nz, my, mgalx + 2), 7.0, dtype=torch.float64, device="cuda")
+    step.fourx(fou, phys, mgalx, +1)
+    torch.cuda.synchronize()
+    ref = S.fourx_inverse(c, mgalx)                                                 # [z, y, x]
+    got = phys.cpu().numpy()
+    assert relerr(got[..., :mgalx], ref) <= 1e-13 and not got[..., mgalx:].any()
+    p2 = rng.standard_normal((nz, my, mgalx))
+    ph = cuda(np.concatenate([p2, np.zeros((nz, my, 2))], axis=-1))
+    back = torch.zeros_like(fou)
+    step.fourx(back, ph, mgalx, -1)
+    torch.cuda.synchronize()
+    want = np.moveaxis(S.fourx_forward(p2, M), 0, -1)
+    assert relerr(back.cpu().numpy(), want) <= 1e-13
