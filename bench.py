@@ -48,6 +48,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15}   # MI355X dense matrix peaks, FLOP/s
+# HBM bytes per launch of a kernel class at a workload (m, nmodes, nfields), from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs scaled by the same counters over a plain 1 GiB copy
+# (profiles/k_projf_f64_fold_257x256x4_pmc.md: fetch counted 0.5 per byte, write 1.0).
+TRAFFIC = {("fold", 257, 256, 4): 1153.3e6}
 TOL = 1e-10
 
 
@@ -63,7 +67,65 @@ def parse():
                     default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a float64 buffer of MIB MiB (steps times): the known-size copy the HBM "
+                         "traffic counters are scaled by")
+    ap.add_argument("--c3", action="store_true",
+                    help="only time the pseudo-spectral step at 129^2 x 64 modes, mgalx 192: advection3d, "
+                         "then the projection substep (three Helmholtz solves, the Poisson solve, the update)")
     return ap.parse_args()
+
+
+def copy_probe(mib, steps):
+    """A plain device copy of known size; under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per
+    dispatch give the counters' scale (bytes counted per byte moved)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n * 8, "copies": steps}), flush=True)
+
+
+def c3_bench(steps, warmup):
+    """Config 3 (129^2 CGL, 64 modes, mgalx = 192): time of advection3d and of one projection
+    substep on device; the kernel classes come from the library's own hipEvents."""
+    import torch
+    import duct_amd
+    from duct_amd import step, workloads
+    cfg = workloads.CONFIGS[3]
+    torch.cuda.set_device(0)
+    duct_amd.init(0)
+    duct_amd.setup(cfg.m)
+    u, v, w = (torch.from_numpy(x).cuda() for x in workloads.duct_velocity(cfg))
+    adv = [torch.empty_like(u) for _ in range(3)]
+    out = {}
+    for name, fn in (("advection3d", lambda: step.advection(u, v, w, *adv, mgalx=192, alp=cfg.alp)),
+                     ("step", lambda: step.c3_step(u, v, w, nu=0.01, dt=0.1, mgalx=192, alp=cfg.alp))):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        with duct_amd.KernelTimer() as kt:
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / steps
+        out[name] = {"ms": dt * 1e3,
+                     "kernels": {k: {"launches": kt.count[k] / steps, "avg_us": kt.ms[k] / kt.count[k] * 1e3}
+                                 for k in kt.ms if kt.count[k]}}
+    lines, X, M = cfg.m * cfg.m, 192, cfg.nmodes
+    xp = out["advection3d"]["kernels"].get("xprod")
+    if xp:
+        flop = 36.0 * lines * X * M          # dense x-DFTs: 3 inverse (Q = 2M), 6 forward (Re, Im)
+        out["advection3d"]["xprod_dense_dft_tflops"] = flop / (xp["avg_us"] * 1e-6) / 1e12
+        out["advection3d"]["xprod_hbm_gbs"] = 9 * lines * M * 16 / (xp["avg_us"] * 1e-6) / 1e9
+    print(json.dumps({"workload": "pseudo-spectral step, %d^2 CGL x %d modes, mgalx 192" % (cfg.m, M),
+                      "steps": steps, **out}), flush=True)
+    duct_amd.shutdown()
 
 
 def cpu_baseline(cfg, fields, alams, threads):
@@ -104,6 +166,12 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the solve shards over Fourier modes with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe, args.steps)
+        return
+    if args.c3:
+        c3_bench(args.steps, args.warmup)
+        return
     import torch
 
     import duct_amd
@@ -237,7 +305,8 @@ def main():
                    "m": m, "nmodes": M, "nfields": F, "path": chosen,
                    "mixed": {"scheme0": s0, "scheme1": s1, "refinements": niter}},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[unit_peak] / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[unit_peak], "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK[unit_peak],
+                     "traffic": TRAFFIC.get((dom, m, M, F)),
                      "kernel": {"proj": "k_proj (FP64 MFMA projection pass)",
                                 "fold": "k_projf (FP64 MFMA projection pass, even/odd folded basis)",
                                 "lp": "k_proj_lp (%s MFMA projection pass)" % unit_peak,
