@@ -54,6 +54,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
     static_assert(TH >= 1 && NH % 64 == 0, "folded kernel needs NH >= 64");
 
     __shared__ double Bs[2][2][NT * LDB];     // [buffer][row parity]
+    __shared__ double lam_s[LDIV ? 2 * NH : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -131,6 +132,12 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
         }
     }
 
+    double ld_lamc = 0.0;
+    if constexpr (LDIV) {
+        for (int i = threadIdx.x; i < 2 * NH; i += 256) lam_s[i] = a.lam[i];
+        if (src_ok) ld_lamc = a.lam[ld_cp * NH + ld_l] - ld_xlam;
+        __syncthreads();
+    }
     double breg[CIN ? 1 : 2][CIN ? 1 : DPT];
     d2v creg[PPT][4];
 
@@ -159,19 +166,9 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
                     breg[rp][e] = v[0];
                     breg[rp][e + 1] = v[1];
                 }
-            if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
-                const int hh[2] = {he, n >> 1};
-#pragma unroll
-                for (int rp = 0; rp < 2; ++rp)
-#pragma unroll
-                    for (int e = 0; e < DPT; ++e) {
-                        const int k = k0 + e;
-                        if (ld_l < hh[ld_cp] && k < hh[rp] && !(ld_gauge && rp == 0 && k == a.gauge))
-                            breg[rp][e] = breg[rp][e] / (a.lam[rp * NH + k] + a.lam[ld_cp * NH + ld_l] - ld_xlam);
-                    }
-            }
         }
     };
+    int st_k0 = 0;                            // first contraction index of the stage being stored
     auto store_b = [&](int buf) {
         if constexpr (CIN) {
 #pragma unroll
@@ -199,7 +196,17 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
 #pragma unroll
             for (int rp = 0; rp < 2; ++rp)
 #pragma unroll
-                for (int e = 0; e < DPT; ++e) Bs[buf][rp][v_dst * LDB + kk0 + e] = breg[rp][e];
+                for (int e = 0; e < DPT; ++e) {
+                    double v = breg[rp][e];
+                    if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
+                        const int k = st_k0 + kk0 + e;
+                        const int hh = rp ? (n >> 1) : he;
+                        const int hc = ld_cp ? (n >> 1) : he;
+                        if (ld_l < hc && k < hh && !(ld_gauge && rp == 0 && k == a.gauge))
+                            v = v / (lam_s[rp * NH + k] + ld_lamc);
+                    }
+                    Bs[buf][rp][v_dst * LDB + kk0 + e] = v;
+                }
         }
     };
 
@@ -212,7 +219,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
         const int mtg = (rp * NH + idx0 + (mt % TH) * 16) / 16;
         Ab[mt] = (const d2v*)a.Afrag + (size_t)mtg * (NH / 8) * 64 + lane;
     }
-    d2v acur[TM][2], anext[TM][2];
+    d2v acur[TM][2];
     auto load_a = [&](int ks, d2v (&dst)[TM][2]) {
 #pragma unroll
         for (int mt = 0; mt < TM; ++mt)
@@ -228,14 +235,15 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
 
     load_b(0);
     load_a(0, acur);
+    st_k0 = 0;
     store_b(0);
     __syncthreads();
 
+#pragma unroll 2
     for (int ks = 0; ks < NK; ++ks) {
         const int buf = ks & 1;
         if (ks + 1 < NK) {
             load_b(ks + 1);
-            load_a(ks + 1, anext);
         }
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
@@ -252,14 +260,14 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
                 for (int nt = 0; nt < 4; ++nt)
                     acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bf[mt / TH][nt], acc[mt][nt], 0, 0, 0);
             }
+            if ((kk & 1) && ks + 1 < NK) {   // this half of the A fragments is dead: fetch the next stage's
+#pragma unroll
+                for (int mt = 0; mt < TM; ++mt) acur[mt][kk >> 1] = Ab[mt][(size_t)((ks + 1) * 2 + (kk >> 1)) * 64];
+            }
         }
         if (ks + 1 < NK) {
+            st_k0 = (ks + 1) * S::KT;
             store_b(buf ^ 1);
-#pragma unroll
-            for (int mt = 0; mt < TM; ++mt) {
-                acur[mt][0] = anext[mt][0];
-                acur[mt][1] = anext[mt][1];
-            }
         }
         __syncthreads();
     }
