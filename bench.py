@@ -72,7 +72,8 @@ def parse():
                          "traffic counters are scaled by")
     ap.add_argument("--c3", action="store_true",
                     help="only time the pseudo-spectral step at 129^2 x 64 modes, mgalx 192: advection3d, "
-                         "then the projection substep (three Helmholtz solves, the Poisson solve, the update)")
+                         "the backward-Euler projection substep, and one full RK3 time step (three substeps "
+                         "of duct_rk3_substep_dev_)")
     return ap.parse_args()
 
 
@@ -102,9 +103,15 @@ def c3_bench(steps, warmup):
     duct_amd.setup(cfg.m)
     u, v, w = (torch.from_numpy(x).cuda() for x in workloads.duct_velocity(cfg))
     adv = [torch.empty_like(u) for _ in range(3)]
+    # the RK3 state evolves in place: a copy of the velocity, zero pressure and advection
+    ru, rv, rw = u.clone(), v.clone(), w.clone()
+    rp = torch.zeros_like(u)
+    radv = [torch.zeros_like(u) for _ in range(3)]
     out = {}
     for name, fn in (("advection3d", lambda: step.advection(u, v, w, *adv, mgalx=192, alp=cfg.alp)),
-                     ("step", lambda: step.c3_step(u, v, w, nu=0.01, dt=0.1, mgalx=192, alp=cfg.alp))):
+                     ("step", lambda: step.c3_step(u, v, w, nu=0.01, dt=0.1, mgalx=192, alp=cfg.alp)),
+                     ("rk3_step", lambda: step.rk3_step(ru, rv, rw, rp, radv, fnu=0.01, dt=0.01, mgalx=192,
+                                                        alp=cfg.alp))):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()

@@ -39,6 +39,9 @@ struct DerivyzArgs {
 hipError_t launch_derivyz(const DerivyzArgs& a, hipStream_t st);
 // Chebyshev transform matrices (column-major m x m): fwd = phys -> coefficients, inv = the inverse.
 void build_cheb_transforms(int m, std::vector<double>& fwd, std::vector<double>& inv);
+// The collocation first and second derivative matrices of the m CGL nodes, column-major m x m,
+// correctly rounded from an extended-precision evaluation (the derivative products' matrices).
+void build_colloc_matrices(int m, std::vector<double>& d1, std::vector<double>& d2);
 void build_xdft(int M, int X, std::vector<double>& einv, std::vector<double>& fwd);
 
 }  // namespace duct

@@ -287,6 +287,41 @@ void build_cheb_transforms(int m, std::vector<double>& fwd, std::vector<double>&
         }
 }
 
+void build_colloc_matrices(int m, std::vector<double>& d1, std::vector<double>& d2)
+{
+    // D_ik = (c_i / c_k) (-1)^(i+k) / (x_i - x_k), x_j = -cos(pi j / N) (CHQZ (2.4.31), the matrix of
+    // pseudomat.f90:90-107), evaluated in extended precision: x_i - x_k from the product
+    // 2 sin(pi (i+k) / 2N) sin(pi (i-k) / 2N), which has no cancellation, the diagonal from the
+    // zero row sums, D2 = D D accumulated in extended precision; both rounded once to double.
+    // The product of the double-rounded textbook entries loses ~1e-5 relative in second derivatives
+    // at m = 129; these matrices reproduce the interpolant's derivatives to ~1e-10 of the rows' scale.
+    const int N = m - 1;
+    typedef long double X;
+    const X pi = 3.141592653589793238462643383279502884L;
+    std::vector<X> D((size_t)m * m, 0.0L);
+    for (int i = 0; i < m; ++i) {
+        X rs = 0.0L;
+        for (int k = 0; k < m; ++k) {
+            if (k == i) continue;
+            const X ci = (i == 0 || i == N) ? 2.0L : 1.0L, ck = (k == 0 || k == N) ? 2.0L : 1.0L;
+            const X dx = 2.0L * sinl(pi * (X)(i + k) / (2.0L * N)) * sinl(pi * (X)(i - k) / (2.0L * N));
+            const X v = (ci / ck) * (((i + k) & 1) ? -1.0L : 1.0L) / dx;
+            D[i + (size_t)k * m] = v;
+            rs += v;
+        }
+        D[i + (size_t)i * m] = -rs;
+    }
+    d1.assign((size_t)m * m, 0.0);
+    d2.assign((size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) {
+            X acc = 0.0L;
+            for (int k = 0; k < m; ++k) acc += D[i + (size_t)k * m] * D[k + (size_t)j * m];
+            d1[i + (size_t)j * m] = (double)D[i + (size_t)j * m];
+            d2[i + (size_t)j * m] = (double)acc;
+        }
+}
+
 void build_xdft(int M, int X, std::vector<double>& einv, std::vector<double>& fwd)
 {
     const int Q = 2 * M;

@@ -19,6 +19,7 @@
 #include "mixed_fold.h"
 #include "proj.h"
 #include "proj_lp.h"
+#include "rk3.h"
 
 using namespace duct;
 
@@ -87,6 +88,9 @@ struct Lib {
     unsigned long long* stats = nullptr;  // residual maxima of the mixed solve
     double* dcfrag[2] = {nullptr, nullptr}; // DC and DC^T, FP64 fragment layout (derivative products)
     double* tfrag[2] = {nullptr, nullptr};  // Chebyshev transforms phys->cheb, cheb->phys (transyz)
+    double* d2frag = nullptr;             // D2 = DC DC, FP64 fragment layout (laplacian3d)
+    d2v* rk = nullptr;                    // Runge-Kutta substep work fields: h1..h3, advnew1..3, phi
+    size_t rk_bytes = 0;
     int dc_m = 0, dc_np = 0;
     int fold = 0;                         // 1: Dirichlet groups use the folded passes, 2: Neumann too
     double* xdft[2] = {nullptr, nullptr}; // advection x-DFTs (inverse, forward), fragment layout
@@ -856,6 +860,73 @@ int solve_fields(int nf, const duct_devptr_t* rhs, const duct_devptr_t* sol, con
     return OK;
 }
 
+// The advection x-DFT tables for (M, X), rebuilt when either changes.
+int ensure_xdft(int M, int X)
+{
+    if (L.xdft_M == M && L.xdft_X == X) return OK;
+    std::vector<double> ei, fw;
+    build_xdft(M, X, ei, fw);
+    HIPCHK(hipStreamSynchronize(L.stream));
+    free_ptr(L.xdft[0]);
+    free_ptr(L.xdft[1]);
+    L.xdft_M = L.xdft_X = 0;
+    HIPCHK(hipMalloc((void**)&L.xdft[0], ei.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(L.xdft[0], ei.data(), ei.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void**)&L.xdft[1], fw.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(L.xdft[1], fw.data(), fw.size() * sizeof(double), hipMemcpyHostToDevice));
+    L.xdft_M = M;
+    L.xdft_X = X;
+    return OK;
+}
+
+// advection3d (operators.F:9-276) of device fields: the x-transforms and the six products in
+// k_xprod, then the nine derivative terms; the caller has checked the sizes.
+int advection_run(const d2v* const in[3], d2v* const out[3], int mm, int M, int X, double alp, double aspect)
+{
+    int st = ensure_xdft(M, X);
+    if (st) return st;
+    const size_t field = (size_t)M * mm * mm;
+    if ((st = ensure_buf(L.prods, L.prods_bytes, 6 * field * sizeof(d2v)))) return st;
+    XprodArgs xa{};
+    for (int i = 0; i < 3; ++i) xa.in[i] = in[i];
+    for (int p = 0; p < 6; ++p) xa.out[p] = L.prods + p * field;
+    xa.Einv = L.xdft[0];
+    xa.Fwd = L.xdft[1];
+    xa.m = mm; xa.M = M; xa.X = X;
+    {
+        Timed t(PK_XPROD);
+        HIPCHK(launch_xprod(xa, L.stream));
+    }
+    // adv1 = d(uu)/dx + d(uv)/dy + d(uw)/dz, adv2 = d(vv)/dy + d(uv)/dx + d(vw)/dz,
+    // adv3 = d(ww)/dz + d(uw)/dx + d(vw)/dy (operators.F:91-272)
+    const d2v* P[6];
+    for (int p = 0; p < 6; ++p) P[p] = L.prods + p * field;
+    enum { UU, VV, WW, UV, UW, VW };
+    struct Term { int out, prod; char dir; };
+    const Term terms[9] = {{0, UU, 'x'}, {0, UV, 'y'}, {0, UW, 'z'}, {1, VV, 'y'}, {1, UV, 'x'}, {1, VW, 'z'},
+                           {2, WW, 'z'}, {2, UW, 'x'}, {2, VW, 'y'}};
+    for (int t = 0; t < 9; ++t) {
+        const Term& T = terms[t];
+        const double beta = (t % 3) ? 1.0 : 0.0;
+        d2v* o = out[T.out];
+        Timed tm(PK_DERIV);
+        if (T.dir == 'x') {
+            HIPCHK(launch_derivx(P[T.prod], o, mm, M, 0, alp, 1.0, beta, L.stream));
+        } else {
+            CollocArgs a{};
+            a.g.nf = 1; a.g.nmodes = M; a.g.m = mm; a.g.n = mm - 2;
+            a.g.in[0] = P[T.prod];
+            a.g.out[0] = o;
+            a.Dfrag = L.dcfrag[0];
+            a.NP = L.dc_np;
+            a.scal = T.dir == 'z' ? 1.0 / aspect : 1.0;
+            a.beta = beta;
+            HIPCHK(launch_colloc(T.dir == 'y' ? CL_DY : CL_DZ, a, L.stream));
+        }
+    }
+    return OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -885,7 +956,10 @@ int duct_shutdown_(void)
     free_ptr(L.dcfrag[1]);
     free_ptr(L.tfrag[0]);
     free_ptr(L.tfrag[1]);
+    free_ptr(L.d2frag);
     L.dc_m = L.dc_np = 0;
+    free_ptr(L.rk);
+    L.rk_bytes = 0;
     free_ptr(L.xdft[0]);
     free_ptr(L.xdft[1]);
     L.xdft_M = L.xdft_X = 0;
@@ -988,15 +1062,17 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
         build_frag(T.data(), n, n, NP, bwd.data());
         if ((st = up(B.AbwdT, bwd.data(), bwd.size()))) return st;
     }
-    {   // collocation derivative matrix DC and its transpose, padded to a multiple of 16
+    {   // collocation derivative matrices of the derivative products (DC, DC^T, D2), padded to a
+        // multiple of 16: built here from m in extended precision, not from the caller's DC
         const int MP = (m + 15) / 16 * 16;
-        std::vector<double> f((size_t)MP * MP), T((size_t)m * m);
+        std::vector<double> f((size_t)MP * MP), T((size_t)m * m), d1, d2;
+        build_colloc_matrices(m, d1, d2);
         free_ptr(L.dcfrag[0]);
         free_ptr(L.dcfrag[1]);
-        build_frag(DC, m, m, MP, f.data());
+        build_frag(d1.data(), m, m, MP, f.data());
         if ((st = up(L.dcfrag[0], f.data(), f.size()))) return st;
         for (int j = 0; j < m; ++j)
-            for (int i = 0; i < m; ++i) T[i + (size_t)j * m] = DC[j + (size_t)i * m];
+            for (int i = 0; i < m; ++i) T[i + (size_t)j * m] = d1[j + (size_t)i * m];
         build_frag(T.data(), m, m, MP, f.data());
         if ((st = up(L.dcfrag[1], f.data(), f.size()))) return st;
         std::vector<double> tf, ti;
@@ -1007,6 +1083,10 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
         if ((st = up(L.tfrag[0], f.data(), f.size()))) return st;
         build_frag(ti.data(), m, m, MP, f.data());
         if ((st = up(L.tfrag[1], f.data(), f.size()))) return st;
+        // D2 = DC DC (pseudomat.f90:110), the full second-derivative matrix of laplacian3d
+        free_ptr(L.d2frag);
+        build_frag(d2.data(), m, m, MP, f.data());
+        if ((st = up(L.d2frag, f.data(), f.size()))) return st;
         L.dc_m = m;
         L.dc_np = MP;
     }
@@ -1443,25 +1523,6 @@ int duct_helm3d_adjoint_batch_dev_(const int* nfields, const duct_devptr_t* solb
     return OK;
 }
 
-// The advection x-DFT tables for (M, X), rebuilt when either changes.
-int ensure_xdft(int M, int X)
-{
-    if (L.xdft_M == M && L.xdft_X == X) return OK;
-    std::vector<double> ei, fw;
-    build_xdft(M, X, ei, fw);
-    HIPCHK(hipStreamSynchronize(L.stream));
-    free_ptr(L.xdft[0]);
-    free_ptr(L.xdft[1]);
-    L.xdft_M = L.xdft_X = 0;
-    HIPCHK(hipMalloc((void**)&L.xdft[0], ei.size() * sizeof(double)));
-    HIPCHK(hipMemcpy(L.xdft[0], ei.data(), ei.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc((void**)&L.xdft[1], fw.size() * sizeof(double)));
-    HIPCHK(hipMemcpy(L.xdft[1], fw.data(), fw.size() * sizeof(double), hipMemcpyHostToDevice));
-    L.xdft_M = M;
-    L.xdft_X = X;
-    return OK;
-}
-
 int duct_fourx_dev_(const duct_devptr_t* fou, const duct_devptr_t* phys, const int* my, const int* nz,
                     const int* nmodes, const int* mgalx, const int* iopt)
 {
@@ -1509,47 +1570,9 @@ int duct_advection_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const du
     if (!(*aspect > 0.0)) return fail(ERR_ARG, "aspect must be positive");
     if (!L.dcfrag[0] || L.dc_m != mm)
         return fail(ERR_NOT_SETUP, "no collocation matrix for m=%d (duct_setup_ builds it)", mm);
-    if ((st = ensure_xdft(M, X))) return st;
-    const size_t field = (size_t)M * mm * mm;
-    if ((st = ensure_buf(L.prods, L.prods_bytes, 6 * field * sizeof(d2v)))) return st;
-    XprodArgs xa{};
-    for (int i = 0; i < 3; ++i) xa.in[i] = (const d2v*)in[i];
-    for (int p = 0; p < 6; ++p) xa.out[p] = L.prods + p * field;
-    xa.Einv = L.xdft[0];
-    xa.Fwd = L.xdft[1];
-    xa.m = mm; xa.M = M; xa.X = X;
-    {
-        Timed t(PK_XPROD);
-        HIPCHK(launch_xprod(xa, L.stream));
-    }
-    // adv1 = d(uu)/dx + d(uv)/dy + d(uw)/dz, adv2 = d(vv)/dy + d(uv)/dx + d(vw)/dz,
-    // adv3 = d(ww)/dz + d(uw)/dx + d(vw)/dy (operators.F:91-272)
-    const d2v* P[6];
-    for (int p = 0; p < 6; ++p) P[p] = L.prods + p * field;
-    enum { UU, VV, WW, UV, UW, VW };
-    struct Term { int out, prod; char dir; };
-    const Term terms[9] = {{0, UU, 'x'}, {0, UV, 'y'}, {0, UW, 'z'}, {1, VV, 'y'}, {1, UV, 'x'}, {1, VW, 'z'},
-                           {2, WW, 'z'}, {2, UW, 'x'}, {2, VW, 'y'}};
-    for (int t = 0; t < 9; ++t) {
-        const Term& T = terms[t];
-        const double beta = (t % 3) ? 1.0 : 0.0;
-        d2v* o = (d2v*)out[T.out];
-        Timed tm(PK_DERIV);
-        if (T.dir == 'x') {
-            HIPCHK(launch_derivx(P[T.prod], o, mm, M, 0, *alp, 1.0, beta, L.stream));
-        } else {
-            CollocArgs a{};
-            a.g.nf = 1; a.g.nmodes = M; a.g.m = mm; a.g.n = mm - 2;
-            a.g.in[0] = P[T.prod];
-            a.g.out[0] = o;
-            a.Dfrag = L.dcfrag[0];
-            a.NP = L.dc_np;
-            a.scal = T.dir == 'z' ? 1.0 / *aspect : 1.0;
-            a.beta = beta;
-            HIPCHK(launch_colloc(T.dir == 'y' ? CL_DY : CL_DZ, a, L.stream));
-        }
-    }
-    return OK;
+    const d2v* vin[3] = {(const d2v*)in[0], (const d2v*)in[1], (const d2v*)in[2]};
+    d2v* vout[3] = {(d2v*)out[0], (d2v*)out[1], (d2v*)out[2]};
+    return advection_run(vin, vout, mm, M, X, *alp, *aspect);
 }
 
 int duct_transyz_dev_(const duct_devptr_t* in, const duct_devptr_t* out, const int* m, const int* nmodes,
@@ -1652,6 +1675,140 @@ int duct_derivx_dev_(const duct_devptr_t* in, const duct_devptr_t* out, const in
     if ((st = check_common(*m, *nmodes))) return st;
     Timed t(PK_DERIV);
     HIPCHK(launch_derivx((const d2v*)*in, (d2v*)*out, *m, *nmodes, *ib, *alp, *scal, *beta, L.stream));
+    return OK;
+}
+
+// One Runge-Kutta substep of the velocity and pressure, main1.f90:384-655 (items i-xi),
+// with the low-storage coefficients of mod.f90:51-58.
+int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const duct_devptr_t* w,
+                          const duct_devptr_t* p, const duct_devptr_t* adv1, const duct_devptr_t* adv2,
+                          const duct_devptr_t* adv3, const int* m, const int* nmodes, const int* mgalx,
+                          const double* alp, const double* aspect, const double* fnu, const double* dt, const int* krk,
+                          const double* dpdx)
+{
+    if (!u || !v || !w || !p || !adv1 || !adv2 || !adv3 || !m || !nmodes || !mgalx || !alp || !aspect || !fnu || !dt ||
+        !krk || !dpdx)
+        return fail(ERR_ARG, "null argument to duct_rk3_substep_dev_");
+    const duct_devptr_t f[7] = {*u, *v, *w, *p, *adv1, *adv2, *adv3};
+    for (int i = 0; i < 7; ++i) {
+        if (!f[i]) return fail(ERR_ARG, "null device pointer (argument %d)", i + 1);
+        for (int j = 0; j < i; ++j)
+            if (f[i] == f[j]) return fail(ERR_ARG, "arguments %d and %d alias", j + 1, i + 1);
+    }
+    if (*krk < 1 || *krk > 3) return fail(ERR_ARG, "krk=%d outside [1, 3]", *krk);
+    if (!(*fnu > 0.0) || !(*dt > 0.0) || !(*aspect > 0.0)) return fail(ERR_ARG, "fnu, dt and aspect must be positive");
+    int st = ensure_init();
+    if (st) return st;
+    const int mm = *m, M = *nmodes, X = *mgalx;
+    if ((st = check_common(mm, M))) return st;
+    if (M % 16 || M > kXprodMaxM || X % 16 || X > kXprodMaxX || X < 3 * M)
+        return fail(ERR_UNSUPPORTED, "the substep's advection needs nmodes %% 16 == 0, nmodes <= %d, mgalx %% 16 == 0, "
+                    "3 nmodes <= mgalx <= %d (got %d, %d)", kXprodMaxM, kXprodMaxX, M, X);
+    if (!L.dcfrag[0] || !L.d2frag || L.dc_m != mm)
+        return fail(ERR_NOT_SETUP, "no collocation matrices for m=%d (duct_setup_ builds them)", mm);
+    for (int b = 0; b < 2; ++b)
+        if (!L.basis[b].ready || L.basis[b].m != mm)
+            return fail(ERR_NOT_SETUP, "bound '%c' not set up for m=%d", b ? 'n' : 'd', mm);
+    // Spalart-Moser-Rogers coefficients (mod.f90:51-58)
+    static const double akkc[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
+    static const double bkkc[3] = {37.0 / 160.0, 5.0 / 24.0, 1.0 / 6.0};
+    static const double gkkc[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+    static const double rkkc[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+    const int k = *krk - 1;
+    const double a = akkc[k], b = bkkc[k], nu = *fnu, ab = a + b, asp = *aspect;
+    const size_t field = (size_t)M * mm * mm;
+    if ((st = ensure_buf(L.rk, L.rk_bytes, 7 * field * sizeof(d2v)))) return st;
+    d2v* h[3] = {L.rk, L.rk + field, L.rk + 2 * field};
+    d2v* advnew[3] = {L.rk + 3 * field, L.rk + 4 * field, L.rk + 5 * field};
+    d2v* phi = L.rk + 6 * field;
+    d2v* U[3] = {(d2v*)*u, (d2v*)*v, (d2v*)*w};
+    d2v* P = (d2v*)*p;
+    d2v* ADV[3] = {(d2v*)*adv1, (d2v*)*adv2, (d2v*)*adv3};
+    auto colloc = [&](const d2v* in, d2v* out, const double* frag, char dir, double scal) -> int {
+        CollocArgs c{};
+        c.g.nf = 1; c.g.nmodes = M; c.g.m = mm; c.g.n = mm - 2;
+        c.g.in[0] = in;
+        c.g.out[0] = out;
+        c.Dfrag = frag;
+        c.NP = L.dc_np;
+        c.scal = scal;
+        c.beta = 1.0;
+        Timed t(PK_DERIV);
+        HIPCHK(launch_colloc(dir == 'y' ? CL_DY : CL_DZ, c, L.stream));
+        return OK;
+    };
+    // (ii) adv(u^{k-1}) into advnew
+    const d2v* vin[3] = {U[0], U[1], U[2]};
+    if ((st = advection_run(vin, advnew, mm, M, X, *alp, asp))) return st;
+    // (i), (iii)-(vii): h = -1/(nu b) [-rho adv_old - gam adv + nu a lap(u) + u/dt - (a+b) grad p - (a+b) dpdx]
+    const double s = -1.0 / (nu * b);
+    {
+        Rk3RhsArgs r{};
+        for (int c = 0; c < 3; ++c) {
+            r.u[c] = U[c];
+            r.adv[c] = ADV[c];
+            r.advnew[c] = advnew[c];
+            r.h[c] = h[c];
+        }
+        r.p = P;
+        r.mm = (long)mm * mm;
+        r.nmodes = M;
+        r.mode0 = 0;
+        r.use_old = k > 0;
+        r.alp = *alp;
+        r.s = s;
+        r.rho = rkkc[k];
+        r.gam = gkkc[k];
+        r.clap = nu * a;
+        r.rdt = 1.0 / *dt;
+        r.cp = -ab;
+        r.cdp = -ab * *dpdx;
+        Timed t(PK_DERIV);
+        HIPCHK(launch_rk3_rhs(r, L.stream));
+    }
+    for (int c = 0; c < 3; ++c) {   // laplacian3d's y and z parts, scaled as in (vii)
+        if ((st = colloc(U[c], h[c], L.d2frag, 'y', s * nu * a))) return st;
+        if ((st = colloc(U[c], h[c], L.d2frag, 'z', s * nu * a / (asp * asp)))) return st;
+    }
+    if ((st = colloc(P, h[1], L.dcfrag[0], 'y', -s * ab))) return st;        // grad3d, y and z
+    if ((st = colloc(P, h[2], L.dcfrag[0], 'z', -s * ab / asp))) return st;
+    // (viii) u*, v*, w*: Dirichlet Helmholtz, alambda = 1/(b nu dt)
+    {
+        const double helco = 1.0 / (b * nu * *dt);
+        const duct_devptr_t rhs[3] = {(duct_devptr_t)h[0], (duct_devptr_t)h[1], (duct_devptr_t)h[2]};
+        const duct_devptr_t sol[3] = {*u, *v, *w};
+        const double al[3] = {helco, helco, helco};
+        if ((st = solve_fields(3, rhs, sol, "ddd", al, mm, M, 0, *alp, BND_DIR_ZERO))) return st;
+    }
+    // (ix) div3d(u*, v*, w*) / ((a+b) dt) -> h1
+    const double sd = 1.0 / (ab * *dt);
+    {
+        Timed t(PK_DERIV);
+        HIPCHK(launch_xderiv(U[0], h[0], (long)mm * mm, M, 0, 1, *alp, sd, 0.0, L.stream));
+    }
+    if ((st = colloc(U[1], h[0], L.dcfrag[0], 'y', sd))) return st;
+    if ((st = colloc(U[2], h[0], L.dcfrag[0], 'z', sd / asp))) return st;
+    // (x) phi: Neumann Poisson
+    {
+        const duct_devptr_t rhs[1] = {(duct_devptr_t)h[0]};
+        const duct_devptr_t sol[1] = {(duct_devptr_t)phi};
+        const double al[1] = {0.0};
+        if ((st = solve_fields(1, rhs, sol, "n", al, mm, M, 0, *alp, BND_DIR_ZERO))) return st;
+    }
+    // (xi) update_uvwp: rhsphi = (a+b) dt nu lap(phi) -> h1; p += phi - rhsphi; u -= dt (a+b) grad phi
+    const double sl = ab * *dt * nu, su = -*dt * ab;
+    {
+        Timed t(PK_DERIV);
+        HIPCHK(launch_xderiv(phi, h[0], (long)mm * mm, M, 0, 2, *alp, sl, 0.0, L.stream));
+    }
+    if ((st = colloc(phi, h[0], L.d2frag, 'y', sl))) return st;
+    if ((st = colloc(phi, h[0], L.d2frag, 'z', sl / (asp * asp)))) return st;
+    {
+        Timed t(PK_DERIV);
+        HIPCHK(launch_rk3_update(P, U[0], phi, h[0], (long)mm * mm, M, 0, *alp, su, L.stream));
+    }
+    if ((st = colloc(phi, U[1], L.dcfrag[0], 'y', su))) return st;
+    if ((st = colloc(phi, U[2], L.dcfrag[0], 'z', su / asp))) return st;
     return OK;
 }
 

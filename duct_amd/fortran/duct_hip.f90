@@ -11,7 +11,8 @@ module duct_hip
   public :: duct_init, duct_shutdown, duct_setup, duct_helm3d, duct_helm3d_dev, &
             duct_helm3d_batch_dev, duct_int2d3d, duct_solvediag, duct_check, duct_set_fold, &
             duct_helm3d_mixed_batch_dev, duct_helm3d_adjoint_batch_dev, duct_colloc_deriv_dev, &
-            duct_derivx_dev, duct_advection_dev, duct_transyz_dev, duct_derivyz_dev, duct_fourx_dev
+            duct_derivx_dev, duct_advection_dev, duct_transyz_dev, duct_derivyz_dev, duct_fourx_dev, &
+            duct_rk3_substep_dev
 
   interface
     integer(c_int) function duct_init(device) bind(C, name="duct_init_")
@@ -141,6 +142,15 @@ module duct_hip
       integer(c_size_t), intent(in) :: in, out
       integer(c_int), intent(in) :: m, nmodes, ib
       real(c_double), intent(in) :: alp, scal, beta
+    end function
+
+    ! one RK3 substep of u, v, w, p (main1.f90:384-655)
+    integer(c_int) function duct_rk3_substep_dev(u, v, w, p, adv1, adv2, adv3, m, nmodes, mgalx, alp, aspect, &
+                                                 fnu, dt, krk, dpdx) bind(C, name="duct_rk3_substep_dev_")
+      import :: c_int, c_size_t, c_double
+      integer(c_size_t), intent(in) :: u, v, w, p, adv1, adv2, adv3
+      integer(c_int), intent(in) :: m, nmodes, mgalx, krk
+      real(c_double), intent(in) :: alp, aspect, fnu, dt, dpdx
     end function
 
     ! fourx (fourier.F:1-115) on the z-cut layouts

@@ -167,6 +167,26 @@ int duct_colloc_deriv_dev_(const duct_devptr_t *in, const duct_devptr_t *out, co
 int duct_fourx_dev_(const duct_devptr_t *fou, const duct_devptr_t *phys, const int *my, const int *nz,
                     const int *nmodes, const int *mgalx, const int *iopt);
 
+/* One substep krk (1..3) of the low-storage RK3 for velocity and pressure, the
+ * loop body of src/main1.f90:384-655 (items i-xi; icase = 1, no CMASSFLOW, forcing,
+ * FILTER or temperature), on complex(8) device fields (m, m, nmodes), modes from 0:
+ *   h    = -rkkc adv - gkkc advection3d(u) + fnu akkc lap(u) + u/dt
+ *          - (akkc+bkkc) grad(p) - (akkc+bkkc) dpdx [x-momentum, mode 0]; h *= -1/(fnu bkkc)
+ *   u,v,w <- helm3d(h, 'd', 1/(bkkc fnu dt));  phi = helm3d(div3d(u)/((akkc+bkkc) dt), 'n', 0)
+ *   update_uvwp (operators.F:503-605): p += phi - (akkc+bkkc) dt fnu lap(phi),
+ *                                       u -= dt (akkc+bkkc) grad(phi)
+ * with the coefficients of mod.f90:51-58.  u, v, w, p are updated in place; adv1..3
+ * hold advection3d of the previous substep's velocity on entry (read when krk > 1)
+ * and of this substep's input velocity on exit.  dpdx is the caller's driving
+ * gradient (get_dpdx, operators.F:2478-2522: -fnu dpval(aspect)); 0 for none.
+ * The advection needs nmodes % 16 == 0, nmodes <= 64, mgalx % 16 == 0 and
+ * 3 nmodes <= mgalx <= 192.  Both bounds must be set up for m.                    */
+int duct_rk3_substep_dev_(const duct_devptr_t *u, const duct_devptr_t *v, const duct_devptr_t *w,
+                          const duct_devptr_t *p, const duct_devptr_t *adv1, const duct_devptr_t *adv2,
+                          const duct_devptr_t *adv3, const int *m, const int *nmodes, const int *mgalx,
+                          const double *alp, const double *aspect, const double *fnu, const double *dt,
+                          const int *krk, const double *dpdx);
+
 /* Chebyshev transform of complex(8) fields (m, m, nmodes) along y or z
  * (transyz, src/chebyshev.f90:458-540, through tcheb_fftw, src/tchebw.f:62-150,
  * REDFT00 with the reference's end scalings and no dealiasing): iopt < 0

@@ -148,3 +148,51 @@ def advection3d(u, v, w, mgalx: int, alp: float = 1.0, aspect: float = 1.0, ib: 
     adv3 = adv3 + dy(vw)
     adv2 = adv2 + dz(vw)
     return adv1, adv2, adv3
+
+
+# ---- the remaining operators of the Runge-Kutta substep (operators.F:278-605) ----
+def _dy(f, ider=1, scal=1.0):
+    return transyz(derivyz(transyz(f, "y", -1), "y", ider, scal=scal), "y", +1)
+
+
+def _dz(f, ider=1, scal=1.0, aspect=1.0):
+    return transyz(derivyz(transyz(f, "z", -1), "z", ider, scal=scal, aspect=aspect), "z", +1)
+
+
+def derivx2(u: np.ndarray, alp: float, ib: int = 0, scal: float = 1.0) -> np.ndarray:
+    """fourier.F:117-188, ider = 2: -(alp k)^2."""
+    k = (ib + np.arange(u.shape[0]))[:, None, None]
+    return -scal * (alp * k) ** 2 * u
+
+
+def laplacian3d(u, scal: float, alp: float = 1.0, aspect: float = 1.0, ib: int = 0):
+    """operators.F:278-347, what = 'n': scal (d_xx + d_yy + d_zz) u, the y and z parts through
+    transyz / derivyz(ider = 2) / transyz."""
+    flap = derivx2(u, alp, ib, scal)
+    flap = flap + _dy(u, 2, scal)
+    return flap + _dz(u, 2, scal, aspect)
+
+
+def grad3d(u, scal: float, alp: float = 1.0, aspect: float = 1.0, ib: int = 0):
+    """operators.F:349-425, what = 'n': scal grad(u) -> (dxu, dyu, dzu)."""
+    return derivx(u, alp, ib, scal), _dy(u, 1, scal), _dz(u, 1, scal, aspect)
+
+
+def div3d(u, v, w, scal: float, alp: float = 1.0, aspect: float = 1.0, ib: int = 0):
+    """operators.F:427-501: scal (du/dx + dv/dy + dw/dz)."""
+    div = derivx(u, alp, ib, scal)
+    div = div + _dy(v, 1, scal)
+    return div + _dz(w, 1, scal, aspect)
+
+
+def update_uvwp(u, v, w, p, phi, akk: float, bkk: float, dt: float, fnu: float, alp: float = 1.0,
+                aspect: float = 1.0, ib: int = 0):
+    """operators.F:503-605 without FILTER: p += phi - (a+b) dt nu lap(phi);
+    u_i += -dt (a+b) d_i phi.  Returns (u, v, w, p) as new arrays."""
+    rhsphi = laplacian3d(phi, (akk + bkk) * dt * fnu, alp, aspect, ib)
+    p = p + phi - rhsphi
+    scal = -dt * (akk + bkk)
+    u = u + derivx(phi, alp, ib, scal)
+    v = v + _dy(phi, 1, scal)
+    w = w + _dz(phi, 1, scal, aspect)
+    return u, v, w, p

@@ -41,6 +41,9 @@ program use_duct
   call duct_check(duct_transyz_dev(r(1), s(1), m, nm, 'y', -1_c_int, 1_c_size_t), 'transyz')
   call duct_check(duct_derivyz_dev(r(1), s(1), m, nm, 'z', 1_c_int, 1.0_c_double, 'n', 1.0_c_double, &
        1_c_size_t, 1_c_size_t), 'derivyz')
+  call duct_check(duct_fourx_dev(r(1), s(1), m, m, 16_c_int, 48_c_int, 1_c_int), 'fourx')
+  call duct_check(duct_rk3_substep_dev(r(1), r(1), r(1), r(1), s(1), s(1), s(1), m, 16_c_int, 48_c_int, &
+       1.0_c_double, 1.0_c_double, 0.01_c_double, 0.1_c_double, 1_c_int, 0.0_c_double), 'rk3')
   call duct_check(duct_shutdown(), 'shutdown')
 end program use_duct
 """

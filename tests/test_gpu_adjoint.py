@@ -70,22 +70,32 @@ def test_solve_adjoint_identity(dev, m, M, ib, bound, alam):
 
 @pytest.mark.parametrize("m", [17, 129, 257])
 def test_colloc_deriv_and_derivx(dev, oracle_mod, m):
+    """d/dy, d/dz against the oracle's transform-recurrence route (transyz + derivyz + transyz, the
+    reference's algorithm, itself pinned against exact differentiation); the transposed product
+    through the adjoint identity <D u, g> = <u, D^T g> with the forward product."""
     import torch
     from duct_amd import adjoint
+    from oracle import spectral as S
     ensure(dev, m)
-    DC = oracle_mod.pseudomat(m).DC if m < 257 else dev.solver._state["setup"].DC
     M = 3
     u, g = rnd((M, m, m), m), rnd((M, m, m), m + 1)
+    _, DC = dev.basis.collocation_matrix(m)
+    scale = np.abs(DC).sum(1).max() * np.max(np.abs(u))
+    fwd = {}
     for cdir in ("y", "z"):
-        for trans in (False, True):
-            D = DC.T if trans else DC
-            want = u @ D.T if cdir == "y" else np.einsum("ij,kjl->kil", D, u)
-            out = cuda(g.copy())
-            adjoint.colloc_deriv(cuda(u), out, cdir, trans=trans, scal=0.5, beta=2.0)
-            torch.cuda.synchronize()
-            got = out.cpu().numpy()
-            ref = 0.5 * want + 2.0 * g
-            assert np.max(np.abs(got - ref)) <= 1e-13 * np.abs(D).sum(1).max() * np.max(np.abs(u)), (cdir, trans)
+        d = S.transyz(S.derivyz(S.transyz(u, cdir, -1), cdir, 1), cdir, +1)
+        out = cuda(g.copy())
+        adjoint.colloc_deriv(cuda(u), out, cdir, trans=False, scal=0.5, beta=2.0)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        assert np.max(np.abs(got - (0.5 * d + 2.0 * g))) <= 1e-14 * scale, cdir
+        fwd[cdir] = (got - 2.0 * g) / 0.5
+        out = cuda(np.zeros_like(g))
+        adjoint.colloc_deriv(cuda(g), out, cdir, trans=True, scal=1.0, beta=0.0)
+        torch.cuda.synchronize()
+        lhs = np.vdot(fwd[cdir], g)
+        rhs = np.vdot(u, out.cpu().numpy())
+        assert abs(lhs - rhs) <= 1e-13 * scale * np.abs(g).sum(), cdir
     out = cuda(g.copy())
     adjoint.derivx(cuda(u), out, ib=2, alp=0.5, scal=3.0, beta=-1.0)
     torch.cuda.synchronize()

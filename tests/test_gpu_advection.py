@@ -137,15 +137,18 @@ def test_config3_step(dev, oracle_mod):
         e = relerr(got[c], ref[c])
         print("config 3 step u%d rel.err %.2e" % (c + 1, e), flush=True)
         assert e <= 1e-10
-    # the projected velocity is divergence-free in the interior (collocation)
-    from duct_amd import basis
-    _, DC = basis.collocation_matrix(m)
-    k = np.arange(M)[:, None, None]
-    div = 1j * cfg.alp * k * got[0] + got[1] @ DC.T + np.einsum("ij,mjk->mik", DC, got[2])
+    # the projection leaves the interior divergence the reference's own route leaves: its Poisson
+    # operator D2B comes from the textbook DC (pseudomat.f90:90-110) while div3d and the gradient
+    # take the transform route, and the two second derivatives differ by ~1e-5 relative at
+    # m = 129, so div(u_new) / div(u*) is ~2e-8 there (1e-10 with the textbook DC everywhere)
+    div = S.div3d(got[0], got[1], got[2], 1.0, cfg.alp)      # div3d (operators.F:427-501)
+    div_ref = S.div3d(ref[0], ref[1], ref[2], 1.0, cfg.alp)
     dstar = d * dt
     rel = np.abs(div[:, 1:-1, 1:-1]).max() / np.abs(dstar[:, 1:-1, 1:-1]).max()
-    print("config 3 interior divergence after projection / before: %.2e" % rel, flush=True)
-    assert rel <= 1e-9
+    rel_ref = np.abs(div_ref[:, 1:-1, 1:-1]).max() / np.abs(dstar[:, 1:-1, 1:-1]).max()
+    print("config 3 interior divergence after projection / before: %.2e (oracle route %.2e)" % (rel, rel_ref),
+          flush=True)
+    assert rel <= 1.05 * rel_ref + 1e-12 and rel_ref <= 5e-8
 
 
 @pytest.mark.parametrize("m", [33, 129])
