@@ -51,6 +51,9 @@ struct CollocArgs {
 typedef CollocArgs ResidArgs;
 hipError_t launch_colloc(int mode, const CollocArgs& a, hipStream_t st);
 hipError_t launch_resid(const ResidArgs& a, hipStream_t st);
+// Whole-plane derivative product along y (dirz = 0) or z (dirz = 1) for every plane of a.g:
+// out = beta out + scal (D u | u D^T), one 16-line slab per workgroup (deriv.hip); NP <= 288.
+hipError_t launch_cline(int dirz, const CollocArgs& a, hipStream_t st);
 hipError_t launch_derivx(const d2v* in, d2v* out, int m, int nmodes, int mode0, double alp, double scal, double beta,
                          hipStream_t st);
 

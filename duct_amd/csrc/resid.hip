@@ -1,8 +1,8 @@
 // FP64 collocation-operator products on complex(8) fields, v_mfma_f64_16x16x4f64.
 //
 //   k_colloc<CL_RESID>   r = f - (D2B_y u + u D2B_z^T - xlambda u)   interior (n x n)
-//   k_colloc<CL_DY>      out = beta out + scal (D u)      along y       full plane (m x m)
-//   k_colloc<CL_DZ>      out = beta out + scal (u D^T)    along z       full plane
+//   launch_colloc(CL_DY / CL_DZ): out = beta out + scal (D u | u D^T) on full planes (m x m),
+//                        served by the slab kernel k_cline of deriv.hip
 //
 // The residual of the mixed solve uses D2B, the interior operator the eigenbasis
 // diagonalises (pseudomat.f90:232-240 Dirichlet, :114-128 Neumann), so r is the
@@ -224,6 +224,8 @@ __global__ __launch_bounds__(256, 2) void k_colloc(CollocArgs a, int bpd, long n
 
 hipError_t launch_colloc(int mode, const CollocArgs& a, hipStream_t st)
 {
+    // the single-direction products on whole planes take the slab kernel (deriv.hip)
+    if (mode == CL_DY || mode == CL_DZ) return launch_cline(mode == CL_DZ, a, st);
     const int sz = mode == CL_RESID ? a.g.n : a.g.m;
     const int bpd = (sz + RB - 1) / RB;
     const long nplanes = (long)a.g.nf * a.g.nmodes;
@@ -232,8 +234,6 @@ hipError_t launch_colloc(int mode, const CollocArgs& a, hipStream_t st)
     if (blocks <= 0) return hipSuccess;
     switch (mode) {
         case CL_RESID: hipLaunchKernelGGL(k_colloc<CL_RESID>, dim3((unsigned)blocks), dim3(256), 0, st, a, bpd, nplanes); break;
-        case CL_DY: hipLaunchKernelGGL(k_colloc<CL_DY>, dim3((unsigned)blocks), dim3(256), 0, st, a, bpd, nplanes); break;
-        case CL_DZ: hipLaunchKernelGGL(k_colloc<CL_DZ>, dim3((unsigned)blocks), dim3(256), 0, st, a, bpd, nplanes); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
