@@ -38,6 +38,8 @@ SIGNATURES = {
     "duct_transyz_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _sz]),
     "duct_fourx_dev_": (_i, [_psz, _psz, _pi, _pi, _pi, _pi, _pi]),
     "duct_rk3_substep_dev_": (_i, [_psz] * 7 + [_pi, _pi, _pi, _pd, _pd, _pd, _pd, _pi, _pd]),
+    "duct_setup_temp_": (_i, [_pc, _pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "duct_helm3d_temp_dev_": (_i, [_psz, _psz, _psz, _psz, _pi, _pi, _pi, _pd, _pd]),
     "duct_derivyz_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _pd, _pc, _pd, _sz, _sz]),
     "duct_transpose_dev_": (_i, [_pi, _pi, _pi, _pd, _psz, _psz]),
     "duct_cxyz2yzx_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),

@@ -20,6 +20,7 @@
 #include "proj.h"
 #include "proj_lp.h"
 #include "rk3.h"
+#include "temp.h"
 
 using namespace duct;
 
@@ -64,6 +65,20 @@ struct DevBasis {
     int gauge = -1;          // 0-based eigen-index of the null mode (Neumann)
 };
 
+// Mixed-boundary (temperature) bases: separate y and z eigenbases (pseudomat_temp.f90)
+struct TempDev {
+    bool ready = false;
+    int m = 0, NP = 0;
+    int b[4] = {0, 0, 0, 0};                 // y=-1, y=+1, z=-1, z=+1: 0 = 'd', 1 = 'n'
+    double* Ayi = nullptr;                   // PTYM1, fragment layout (pass 1)
+    double* Azi = nullptr;                   // PTZM1 (pass 2)
+    double* Ay = nullptr;                    // PTY (pass 3)
+    double* Az = nullptr;                    // PTZ (pass 4)
+    double* ly = nullptr;                    // ETY, NP (zero padded)
+    double* lz = nullptr;                    // ETZ
+    double* vec = nullptr;                   // the 8 m wall vectors of TempArgs
+};
+
 struct ProfRec {
     int kind;
     hipEvent_t a, b;
@@ -91,6 +106,9 @@ struct Lib {
     double* d2frag = nullptr;             // D2 = DC DC, FP64 fragment layout (laplacian3d)
     d2v* rk = nullptr;                    // Runge-Kutta substep work fields: h1..h3, advnew1..3, phi
     size_t rk_bytes = 0;
+    TempDev temp;                         // mixed-boundary solve
+    d2v* tfield = nullptr;                // its corrected right-hand side
+    size_t tfield_bytes = 0;
     int dc_m = 0, dc_np = 0;
     int fold = 0;                         // 1: Dirichlet groups use the folded passes, 2: Neumann too
     double* xdft[2] = {nullptr, nullptr}; // advection x-DFTs (inverse, forward), fragment layout
@@ -125,6 +143,14 @@ void free_basis(DevBasis& B)
     for (int p = 0; p < 2; ++p) { free_ptr(B.lq_fwd[p]); free_ptr(B.lq_bwd[p]); free_ptr(B.sfold[p]); }
     free_ptr(B.lq_lam); free_ptr(B.nqf); free_ptr(B.npf);
     B = DevBasis{};
+}
+
+void free_temp()
+{
+    TempDev& T = L.temp;
+    free_ptr(T.Ayi); free_ptr(T.Azi); free_ptr(T.Ay); free_ptr(T.Az);
+    free_ptr(T.ly); free_ptr(T.lz); free_ptr(T.vec);
+    T.ready = false;
 }
 
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -960,6 +986,9 @@ int duct_shutdown_(void)
     L.dc_m = L.dc_np = 0;
     free_ptr(L.rk);
     L.rk_bytes = 0;
+    free_temp();
+    free_ptr(L.tfield);
+    L.tfield_bytes = 0;
     free_ptr(L.xdft[0]);
     free_ptr(L.xdft[1]);
     L.xdft_M = L.xdft_X = 0;
@@ -1809,6 +1838,141 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     }
     if ((st = colloc(phi, U[1], L.dcfrag[0], 'y', su))) return st;
     if ((st = colloc(phi, U[2], L.dcfrag[0], 'z', su / asp))) return st;
+    return OK;
+}
+
+// The GPU-upload tails of pseudomaty_temp / pseudomatz_temp (pseudomat_temp.f90:2-321): the
+// caller's y and z bases (interior n x n, column-major), eigenvalues and scaled derivative
+// matrices (m x m); bounds = the four wall types y=-1, y=+1, z=-1, z=+1.
+int duct_setup_temp_(const char* bounds, const int* m_, const double* PY, const double* PYinv, const double* ETY,
+                     const double* DY, const double* PZ, const double* PZinv, const double* ETZ, const double* DZ,
+                     size_t bounds_len)
+{
+    if (!bounds || !m_ || !PY || !PYinv || !ETY || !DY || !PZ || !PZinv || !ETZ || !DZ)
+        return fail(ERR_ARG, "null argument to duct_setup_temp_");
+    if (bounds_len < 4) return fail(ERR_ARG, "bounds needs 4 characters (y=-1, y=+1, z=-1, z=+1)");
+    int b[4];
+    for (int w = 0; w < 4; ++w) {
+        if (bounds[w] != 'd' && bounds[w] != 'n') return fail(ERR_ARG, "wall %d: bound '%c'", w + 1, bounds[w]);
+        b[w] = bounds[w] == 'n';
+    }
+    const int m = *m_;
+    if (m < kMinM || m > kMaxM) return fail(ERR_UNSUPPORTED, "m=%d outside [%d, %d]", m, kMinM, kMaxM);
+    const int n = m - 2, NP = padded_n(n);
+    int st = ensure_init();
+    if (st) return st;
+    TempDev& T = L.temp;
+    HIPCHK(hipStreamSynchronize(L.stream));
+    free_temp();
+    auto up = [&](double*& dst, const double* src, size_t count) -> int {
+        HIPCHK(hipMalloc((void**)&dst, count * sizeof(double)));
+        HIPCHK(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice));
+        return OK;
+    };
+    std::vector<double> frag((size_t)NP * NP), lp(NP, 0.0), vec((size_t)8 * m, 0.0);
+    build_frag(PYinv, n, n, NP, frag.data());
+    if ((st = up(T.Ayi, frag.data(), frag.size()))) return st;
+    build_frag(PZinv, n, n, NP, frag.data());
+    if ((st = up(T.Azi, frag.data(), frag.size()))) return st;
+    build_frag(PY, n, n, NP, frag.data());
+    if ((st = up(T.Ay, frag.data(), frag.size()))) return st;
+    build_frag(PZ, n, n, NP, frag.data());
+    if ((st = up(T.Az, frag.data(), frag.size()))) return st;
+    for (int i = 0; i < n; ++i) lp[i] = ETY[i];
+    if ((st = up(T.ly, lp.data(), lp.size()))) return st;
+    for (int i = 0; i < n; ++i) lp[i] = ETZ[i];
+    if ((st = up(T.lz, lp.data(), lp.size()))) return st;
+    for (int d = 0; d < 2; ++d) {        // D2 = D D columns 1 and m (pseudomat_temp.f90:98-104), rows 1 and m of D
+        const double* D = d ? DZ : DY;
+        for (int i = 0; i < m; ++i) {
+            double s0 = 0.0, s1 = 0.0;
+            for (int k = 0; k < m; ++k) {
+                s0 += D[i + (size_t)k * m] * D[k];
+                s1 += D[i + (size_t)k * m] * D[k + (size_t)(m - 1) * m];
+            }
+            vec[(size_t)(2 * d) * m + i] = s0;
+            vec[(size_t)(2 * d + 1) * m + i] = s1;
+            vec[(size_t)(4 + 2 * d) * m + i] = D[(size_t)i * m];                 // D(1, i)
+            vec[(size_t)(5 + 2 * d) * m + i] = D[(m - 1) + (size_t)i * m];       // D(m, i)
+        }
+    }
+    if ((st = up(T.vec, vec.data(), vec.size()))) return st;
+    for (int w = 0; w < 4; ++w) T.b[w] = b[w];
+    T.m = m;
+    T.NP = NP;
+    T.ready = true;
+    return OK;
+}
+
+// helm3d_temp (helm3d_temp.f90:2-44, int2d3d_temp.f90, solvediag_temp.f90) on a device field:
+// sol = the mixed-boundary solution of (D_yy + D_zz - xlambda) u = rhs, xlambda = alambda + (alp k)^2,
+// with the wall data bcy [nmodes][2][m] (over z) and bcz [nmodes][2][m] (over y); rhs is not modified.
+int duct_helm3d_temp_dev_(const duct_devptr_t* rhs, const duct_devptr_t* sol, const duct_devptr_t* bcy,
+                          const duct_devptr_t* bcz, const int* m, const int* nmodes, const int* ib, const double* alp,
+                          const double* alambda)
+{
+    if (!rhs || !sol || !bcy || !bcz || !m || !nmodes || !ib || !alp || !alambda)
+        return fail(ERR_ARG, "null argument to duct_helm3d_temp_dev_");
+    if (!*rhs || !*sol || !*bcy || !*bcz) return fail(ERR_ARG, "null device pointer");
+    if (*rhs == *sol) return fail(ERR_ARG, "rhs and sol must not alias (helm3d_temp keeps the rhs)");
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    if (*ib < 0) return fail(ERR_ARG, "ib=%d < 0", *ib);
+    TempDev& T = L.temp;
+    if (!T.ready || T.m != *m) return fail(ERR_NOT_SETUP, "duct_setup_temp_ not called for m=%d", *m);
+    const int NP = T.NP, mm = *m, M = *nmodes;
+    const size_t field = (size_t)M * mm * mm;
+    if ((st = ensure_buf(L.tfield, L.tfield_bytes, field * sizeof(d2v)))) return st;
+    const long planes = (long)M * 2;
+    if ((st = ensure_planar((size_t)planes * NP * NP * sizeof(double)))) return st;
+    TempArgs ta{};
+    ta.rhs = (const d2v*)*rhs;
+    ta.out = L.tfield;
+    ta.bcy = (const double*)*bcy;
+    ta.bcz = (const double*)*bcz;
+    ta.vec = T.vec;
+    ta.m = mm; ta.nmodes = M; ta.mode0 = *ib;
+    ta.by1 = T.b[0]; ta.by2 = T.b[1]; ta.bz1 = T.b[2]; ta.bz2 = T.b[3];
+    {
+        Timed t(PK_BCCORR);
+        HIPCHK(launch_temp_rhs(ta, L.stream));
+    }
+    FieldGroup g{};
+    g.nf = 1; g.nmodes = M; g.mode0 = *ib; g.m = mm; g.n = mm - 2; g.alp = *alp;
+    g.in[0] = L.tfield;
+    g.out[0] = (d2v*)*sol;
+    g.alambda[0] = *alambda;
+    ProjArgs a{};
+    a.g = g;
+    a.lam = T.ly;
+    a.lam2 = T.lz;
+    a.gauge = -1;
+    {   // pass 1: (PTYM1 F)^T
+        Timed t(PK_C2P);
+        a.Afrag = T.Ayi; a.X = nullptr; a.Y = L.plan[0]; a.ncols = (long)M * NP;
+        HIPCHK(launch_proj(NP, PROJ_C2P, a, L.stream));
+    }
+    {   // pass 2: PTYM1 F PTZM1^T ./ (ETY(i) + ETZ(j) - xlambda) (solvediag_temp.f90:84-96)
+        Timed t(PK_P2P_DIV);
+        a.Afrag = T.Azi; a.X = L.plan[0]; a.Y = L.plan[1]; a.ncols = planes * NP;
+        HIPCHK(launch_proj(NP, PROJ_P2P_DIV, a, L.stream));
+    }
+    {   // pass 3: (PTY G)^T
+        Timed t(PK_P2P);
+        a.Afrag = T.Ay; a.X = L.plan[1]; a.Y = L.plan[0]; a.ncols = planes * NP;
+        HIPCHK(launch_proj(NP, PROJ_P2P, a, L.stream));
+    }
+    {   // pass 4: PTY G PTZ^T into the interior of sol
+        Timed t(PK_P2C);
+        a.Afrag = T.Az; a.X = L.plan[0]; a.Y = nullptr; a.ncols = (long)M * NP;
+        HIPCHK(launch_proj(NP, PROJ_P2C, a, L.stream));
+    }
+    {   // edge_value_y / edge_value_z, corners from the rhs
+        Timed t(PK_BOUNDARY);
+        ta.out = (d2v*)*sol;
+        HIPCHK(launch_temp_edges(ta, L.stream));
+    }
     return OK;
 }
 

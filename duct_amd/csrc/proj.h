@@ -33,6 +33,8 @@ struct ProjArgs {
     const double* X;      // planar input, planes of NP x NP column-major
     double* Y;            // planar output
     const double* lam;    // eigenvalues padded with zeros to NP (divide epilogue)
+    const double* lam2;   // when set, the eigenvalues of the other direction: den = lam[i] + lam2[j] - xlambda
+                          // with i the y-eigen index, j the z-eigen index (separate y and z bases)
     long ncols;           // planar: planes*NP columns; complex: nf*nmodes*NP complex columns
     int gauge;            // 0-based eigen-index of the Neumann null mode, -1 if none
     FieldGroup g;

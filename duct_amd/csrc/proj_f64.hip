@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
                 for (int e = 0; e < DPT; ++e) {
                     const int k = k0 + e;
                     if (ld_j < n && k < n && !(ld_gauge && k == a.gauge))
-                        breg[e] = breg[e] / (a.lam[ld_j] + a.lam[k] - ld_xlam);
+                        breg[e] = breg[e] / ((a.lam2 ? a.lam2 : a.lam)[ld_j] + a.lam[k] - ld_xlam);
                 }
             }
         }
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
                     double v = acc[mt][nt][r];
                     if constexpr (MODE == PROJ_P2P_DIV) {
                         if (j < n && row < n && !(gauge_plane && j == a.gauge && row == a.gauge))
-                            v = v / (a.lam[j] + a.lam[row] - xlam);
+                            v = v / (a.lam[j] + (a.lam2 ? a.lam2 : a.lam)[row] - xlam);
                         else
                             v = 0.0;
                     }

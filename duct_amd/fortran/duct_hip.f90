@@ -12,7 +12,7 @@ module duct_hip
             duct_helm3d_batch_dev, duct_int2d3d, duct_solvediag, duct_check, duct_set_fold, &
             duct_helm3d_mixed_batch_dev, duct_helm3d_adjoint_batch_dev, duct_colloc_deriv_dev, &
             duct_derivx_dev, duct_advection_dev, duct_transyz_dev, duct_derivyz_dev, duct_fourx_dev, &
-            duct_rk3_substep_dev
+            duct_rk3_substep_dev, duct_setup_temp, duct_helm3d_temp_dev
 
   interface
     integer(c_int) function duct_init(device) bind(C, name="duct_init_")
@@ -142,6 +142,23 @@ module duct_hip
       integer(c_size_t), intent(in) :: in, out
       integer(c_int), intent(in) :: m, nmodes, ib
       real(c_double), intent(in) :: alp, scal, beta
+    end function
+
+    ! mixed-boundary (temperature) solve: pseudomat_temp uploads and helm3d_temp
+    integer(c_int) function duct_setup_temp(bounds, m, PY, PYinv, ETY, DY, PZ, PZinv, ETZ, DZ, bounds_len) &
+        bind(C, name="duct_setup_temp_")
+      import :: c_int, c_double, c_size_t, c_char
+      character(kind=c_char), intent(in) :: bounds(*)
+      integer(c_int), intent(in) :: m
+      real(c_double), intent(in) :: PY(*), PYinv(*), ETY(*), DY(*), PZ(*), PZinv(*), ETZ(*), DZ(*)
+      integer(c_size_t), value :: bounds_len
+    end function
+    integer(c_int) function duct_helm3d_temp_dev(rhs, sol, bcy, bcz, m, nmodes, ib, alp, alambda) &
+        bind(C, name="duct_helm3d_temp_dev_")
+      import :: c_int, c_size_t, c_double
+      integer(c_size_t), intent(in) :: rhs, sol, bcy, bcz
+      integer(c_int), intent(in) :: m, nmodes, ib
+      real(c_double), intent(in) :: alp, alambda
     end function
 
     ! one RK3 substep of u, v, w, p (main1.f90:384-655)

@@ -44,6 +44,8 @@ program use_duct
   call duct_check(duct_fourx_dev(r(1), s(1), m, m, 16_c_int, 48_c_int, 1_c_int), 'fourx')
   call duct_check(duct_rk3_substep_dev(r(1), r(1), r(1), r(1), s(1), s(1), s(1), m, 16_c_int, 48_c_int, &
        1.0_c_double, 1.0_c_double, 0.01_c_double, 0.1_c_double, 1_c_int, 0.0_c_double), 'rk3')
+  call duct_check(duct_setup_temp('dndn', m, P, Pinv, lam, DC, P, Pinv, lam, DC, 4_c_size_t), 'setup_temp')
+  call duct_check(duct_helm3d_temp_dev(r(1), s(1), r(1), r(1), m, nm, 0_c_int, 1.0_c_double, 2.0_c_double), 'temp')
   call duct_check(duct_shutdown(), 'shutdown')
 end program use_duct
 """
