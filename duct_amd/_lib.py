@@ -40,6 +40,8 @@ SIGNATURES = {
     "duct_rk3_substep_dev_": (_i, [_psz] * 7 + [_pi, _pi, _pi, _pd, _pd, _pd, _pd, _pi, _pd]),
     "duct_setup_temp_": (_i, [_pc, _pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
     "duct_helm3d_temp_dev_": (_i, [_psz, _psz, _psz, _psz, _pi, _pi, _pi, _pd, _pd]),
+    "duct_setup_shen_": (_i, [_pc, _pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "duct_helm3d_shen_dev_": (_i, [_psz, _psz, _pi, _pi, _pi, _pd, _pd, _pc, _sz]),
     "duct_derivyz_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _pd, _pc, _pd, _sz, _sz]),
     "duct_transpose_dev_": (_i, [_pi, _pi, _pi, _pd, _psz, _psz]),
     "duct_cxyz2yzx_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),
@@ -56,7 +58,7 @@ STATUS = {0: "OK", 1: "DUCT_ERR_ARG", 2: "DUCT_ERR_NOT_SETUP", 3: "DUCT_ERR_HIP"
 
 PROF_CLASSES = ["proj_c2p", "proj_p2p_div", "gauge", "proj_p2p", "proj_p2c", "boundary", "bc_correct",
                 "transpose", "invdiag", "lp_c2p", "lp_p2p_div", "lp_p2p", "lp_p2c", "resid", "deriv", "adjoint",
-                "fold_c2p", "fold_p2p_div", "fold_p2p", "fold_p2c", "xprod"]
+                "fold_c2p", "fold_p2p_div", "fold_p2p", "fold_p2c", "xprod", "tridiag"]
 
 
 class DuctError(RuntimeError):

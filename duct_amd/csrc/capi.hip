@@ -21,6 +21,7 @@
 #include "proj_lp.h"
 #include "rk3.h"
 #include "temp.h"
+#include "shen.h"
 
 using namespace duct;
 
@@ -79,6 +80,14 @@ struct TempDev {
     double* vec = nullptr;                   // the 8 m wall vectors of TempArgs
 };
 
+// Shen-Legendre Galerkin matrices of one boundary type (preshen.f90)
+struct ShenDev {
+    bool ready = false;
+    int m = 0;
+    double* frag[4] = {nullptr, nullptr, nullptr, nullptr};   // A_y, A_z, B_y, B_z (m x m padded), k_cline layout
+    double* vec = nullptr;                                     // md, mu, sd, lam (n each)
+};
+
 struct ProfRec {
     int kind;
     hipEvent_t a, b;
@@ -87,7 +96,7 @@ struct ProfRec {
 enum ProfKind { PK_C2P = 0, PK_P2P_DIV = 1, PK_GAUGE = 2, PK_P2P = 3, PK_P2C = 4, PK_BOUNDARY = 5, PK_BCCORR = 6,
                 PK_TRANSPOSE = 7, PK_INVDIAG = 8, PK_LP_C2P = 9, PK_LP_P2P_DIV = 10, PK_LP_P2P = 11,
                 PK_LP_P2C = 12, PK_RESID = 13, PK_DERIV = 14, PK_ADJ = 15, PK_F_C2P = 16,
-                PK_F_P2P_DIV = 17, PK_F_P2P = 18, PK_F_P2C = 19, PK_XPROD = 20 };
+                PK_F_P2P_DIV = 17, PK_F_P2P = 18, PK_F_P2C = 19, PK_XPROD = 20, PK_TRIDIAG = 21 };
 
 struct Lib {
     bool init = false;
@@ -106,6 +115,9 @@ struct Lib {
     double* d2frag = nullptr;             // D2 = DC DC, FP64 fragment layout (laplacian3d)
     d2v* rk = nullptr;                    // Runge-Kutta substep work fields: h1..h3, advnew1..3, phi
     size_t rk_bytes = 0;
+    ShenDev shen[2];                      // Shen-Legendre solve, 'd' and 'n'
+    double* sgam = nullptr;               // its tridiagonal factors
+    size_t sgam_bytes = 0;
     TempDev temp;                         // mixed-boundary solve
     d2v* tfield = nullptr;                // its corrected right-hand side
     size_t tfield_bytes = 0;
@@ -151,6 +163,13 @@ void free_temp()
     free_ptr(T.Ayi); free_ptr(T.Azi); free_ptr(T.Ay); free_ptr(T.Az);
     free_ptr(T.ly); free_ptr(T.lz); free_ptr(T.vec);
     T.ready = false;
+}
+
+void free_shen(ShenDev& S)
+{
+    for (auto& f : S.frag) free_ptr(f);
+    free_ptr(S.vec);
+    S.ready = false;
 }
 
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -987,6 +1006,9 @@ int duct_shutdown_(void)
     free_ptr(L.rk);
     L.rk_bytes = 0;
     free_temp();
+    for (auto& S : L.shen) free_shen(S);
+    free_ptr(L.sgam);
+    L.sgam_bytes = 0;
     free_ptr(L.tfield);
     L.tfield_bytes = 0;
     free_ptr(L.xdft[0]);
@@ -1973,6 +1995,116 @@ int duct_helm3d_temp_dev_(const duct_devptr_t* rhs, const duct_devptr_t* sol, co
         ta.out = (d2v*)*sol;
         HIPCHK(launch_temp_edges(ta, L.stream));
     }
+    return OK;
+}
+
+// preshen (preshen.f90:2-330) for the device: the caller's (host-built) Galerkin matrices of one
+// boundary type.  Ay (n x m) = P^T W, Az (n x m) = W, By (m x n) = R P, Bz (m x n) = R with
+// n = m - 2, column-major; md, mu, sd the mass diagonal, mass (k, k+2) entries and stiffness
+// diagonal; lam the y-eigenvalues of M x = lam S x (entry 0 unused for Neumann).
+int duct_setup_shen_(const char* bound, const int* m_, const double* Ay, const double* Az, const double* By,
+                     const double* Bz, const double* md, const double* mu, const double* sd, const double* lam,
+                     size_t bound_len)
+{
+    (void)bound_len;
+    if (!bound || !m_ || !Ay || !Az || !By || !Bz || !md || !mu || !sd || !lam)
+        return fail(ERR_ARG, "null argument to duct_setup_shen_");
+    const int bidx = bound_index(bound[0]);
+    if (bidx < 0) return fail(ERR_ARG, "preshen: bound should be d or n, got '%c'", bound[0]);
+    const int m = *m_, n = m - 2;
+    if (m < kMinM || m > 16 * 3 * 6) return fail(ERR_UNSUPPORTED, "m=%d outside [%d, 288]", m, kMinM);
+    int st = ensure_init();
+    if (st) return st;
+    ShenDev& S = L.shen[bidx];
+    HIPCHK(hipStreamSynchronize(L.stream));
+    free_shen(S);
+    const int MP = (m + 15) / 16 * 16;
+    std::vector<double> pad((size_t)m * m), frag((size_t)MP * MP);
+    auto up = [&](double*& dst, const double* src, size_t count) -> int {
+        HIPCHK(hipMalloc((void**)&dst, count * sizeof(double)));
+        HIPCHK(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice));
+        return OK;
+    };
+    const double* src[4] = {Ay, Az, By, Bz};
+    for (int i = 0; i < 4; ++i) {
+        std::fill(pad.begin(), pad.end(), 0.0);
+        const bool wide = i < 2;              // n x m (rows padded) or m x n (columns padded)
+        for (int c = 0; c < (wide ? m : n); ++c)
+            for (int r = 0; r < (wide ? n : m); ++r)
+                pad[r + (size_t)c * m] = src[i][r + (size_t)c * (wide ? n : m)];
+        build_frag(pad.data(), m, m, MP, frag.data());
+        if ((st = up(S.frag[i], frag.data(), frag.size()))) return st;
+    }
+    std::vector<double> v((size_t)4 * n);
+    for (int k = 0; k < n; ++k) {
+        v[k] = md[k];
+        v[n + k] = mu[k];
+        v[2 * n + k] = sd[k];
+        v[3 * n + k] = lam[k];
+    }
+    if ((st = up(S.vec, v.data(), v.size()))) return st;
+    S.m = m;
+    S.ready = true;
+    return OK;
+}
+
+// helm3d_shen (helm_shen.f90:2-85, solveshen.f90:5-94) on a device field: sol = the Shen-Galerkin
+// solution of -lap(u) + xlambda u = rhs, xlambda = alambda + (alp (ib+k))^2, nodal values at the
+// CGL nodes in and out (the reference's transyz in / out included); rhs is kept.
+int duct_helm3d_shen_dev_(const duct_devptr_t* rhs, const duct_devptr_t* sol, const int* m, const int* nmodes,
+                          const int* ib, const double* alp, const double* alambda, const char* bound,
+                          size_t bound_len)
+{
+    (void)bound_len;
+    if (!rhs || !sol || !m || !nmodes || !ib || !alp || !alambda || !bound)
+        return fail(ERR_ARG, "null argument to duct_helm3d_shen_dev_");
+    if (!*rhs || !*sol) return fail(ERR_ARG, "null device pointer");
+    if (*rhs == *sol) return fail(ERR_ARG, "rhs and sol must not alias");
+    const int bidx = bound_index(bound[0]);
+    if (bidx < 0) return fail(ERR_ARG, "bound '%c' is neither 'd' nor 'n'", bound[0]);
+    int st = ensure_init();
+    if (st) return st;
+    if ((st = check_common(*m, *nmodes))) return st;
+    ShenDev& S = L.shen[bidx];
+    if (!S.ready || S.m != *m) return fail(ERR_NOT_SETUP, "duct_setup_shen_ not called for '%c', m=%d", bound[0], *m);
+    const int mm = *m, M = *nmodes, n = mm - 2;
+    const size_t field = (size_t)M * mm * mm;
+    if ((st = ensure_buf(L.tfield, L.tfield_bytes, field * sizeof(d2v)))) return st;
+    if ((st = ensure_buf(L.sgam, L.sgam_bytes, field * sizeof(double)))) return st;
+    auto prod = [&](const d2v* in, d2v* out, int which, bool dirz) -> int {
+        CollocArgs c{};
+        c.g.nf = 1; c.g.nmodes = M; c.g.m = mm; c.g.n = n;
+        c.g.in[0] = in;
+        c.g.out[0] = out;
+        c.Dfrag = S.frag[which];
+        c.NP = (mm + 15) / 16 * 16;
+        c.scal = 1.0;
+        c.beta = 0.0;
+        Timed t(PK_DERIV);
+        HIPCHK(launch_cline(dirz, c, L.stream));
+        return OK;
+    };
+    d2v* U = (d2v*)*sol;
+    // F = A_y f A_z^T: Galerkin inner products with phi_p(y) phi_q(z), then inv(P) inv(S) along y
+    if ((st = prod((const d2v*)*rhs, L.tfield, 0, false))) return st;
+    if ((st = prod(L.tfield, U, 1, true))) return st;
+    {   // solV: the tridiagonal systems along z (in place)
+        ShenTriArgs ta{};
+        ta.V = U;
+        ta.gam = L.sgam;
+        ta.md = S.vec;
+        ta.mu = S.vec + n;
+        ta.sd = S.vec + 2 * n;
+        ta.lam = S.vec + 3 * n;
+        ta.alambda = *alambda;
+        ta.alp = *alp;
+        ta.m = mm; ta.nmodes = M; ta.mode0 = *ib; ta.neumann = bidx == 1;
+        Timed t(PK_TRIDIAG);
+        HIPCHK(launch_shen_tri(ta, L.stream));
+    }
+    // u = B_y V B_z^T: back to Shen, Legendre, Chebyshev and the nodes
+    if ((st = prod(U, L.tfield, 2, false))) return st;
+    if ((st = prod(L.tfield, U, 3, true))) return st;
     return OK;
 }
 

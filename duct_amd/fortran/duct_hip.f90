@@ -12,7 +12,8 @@ module duct_hip
             duct_helm3d_batch_dev, duct_int2d3d, duct_solvediag, duct_check, duct_set_fold, &
             duct_helm3d_mixed_batch_dev, duct_helm3d_adjoint_batch_dev, duct_colloc_deriv_dev, &
             duct_derivx_dev, duct_advection_dev, duct_transyz_dev, duct_derivyz_dev, duct_fourx_dev, &
-            duct_rk3_substep_dev, duct_setup_temp, duct_helm3d_temp_dev
+            duct_rk3_substep_dev, duct_setup_temp, duct_helm3d_temp_dev, &
+            duct_setup_shen, duct_helm3d_shen_dev
 
   interface
     integer(c_int) function duct_init(device) bind(C, name="duct_init_")
@@ -142,6 +143,25 @@ module duct_hip
       integer(c_size_t), intent(in) :: in, out
       integer(c_int), intent(in) :: m, nmodes, ib
       real(c_double), intent(in) :: alp, scal, beta
+    end function
+
+    ! Shen-Legendre Galerkin solve (helm3d_shen): preshen uploads and the solve
+    integer(c_int) function duct_setup_shen(bound, m, Ay, Az, By, Bz, md, mu, sd, lam, bound_len) &
+        bind(C, name="duct_setup_shen_")
+      import :: c_int, c_double, c_size_t, c_char
+      character(kind=c_char), intent(in) :: bound(*)
+      integer(c_int), intent(in) :: m
+      real(c_double), intent(in) :: Ay(*), Az(*), By(*), Bz(*), md(*), mu(*), sd(*), lam(*)
+      integer(c_size_t), value :: bound_len
+    end function
+    integer(c_int) function duct_helm3d_shen_dev(rhs, sol, m, nmodes, ib, alp, alambda, bound, bound_len) &
+        bind(C, name="duct_helm3d_shen_dev_")
+      import :: c_int, c_size_t, c_double, c_char
+      integer(c_size_t), intent(in) :: rhs, sol
+      integer(c_int), intent(in) :: m, nmodes, ib
+      real(c_double), intent(in) :: alp, alambda
+      character(kind=c_char), intent(in) :: bound(*)
+      integer(c_size_t), value :: bound_len
     end function
 
     ! mixed-boundary (temperature) solve: pseudomat_temp uploads and helm3d_temp

@@ -167,6 +167,24 @@ int duct_colloc_deriv_dev_(const duct_devptr_t *in, const duct_devptr_t *out, co
 int duct_fourx_dev_(const duct_devptr_t *fou, const duct_devptr_t *phys, const int *my, const int *nz,
                     const int *nmodes, const int *mgalx, const int *iopt);
 
+/* Shen-Legendre spectral-Galerkin solve (-DHELM_SHEN: src/helm_shen.f90:2-85, solveshen.f90:5-513,
+ * preshen.f90:2-535, legendre.f90), an independent cross-check of the collocation solve.
+ * duct_setup_shen_ uploads one boundary type's Galerkin matrices, built by the caller (preshen):
+ * with n = m-2, Ay (n x m) = P^T W and Az (n x m) = W map nodal values to the Galerkin inner
+ * products (f, phi_p phi_q) (the y side through inv(P) inv(S) = P^T of M x = lam S x);
+ * By (m x n) = R P and Bz (m x n) = R map coefficients back to nodal values; md, mu, sd the
+ * mass diagonal, mass (k,k+2) entries and stiffness diagonal, lam the y-eigenvalues (n each).
+ * duct_helm3d_shen_dev_ solves -lap(u) + xlambda u = rhs, xlambda = alambda + (alp (ib+k))^2,
+ * nodal values in and out (rhs kept, unlike the reference, which leaves Chebyshev coefficients
+ * in it): F = Ay f Az^T, the symmetric tridiagonal systems (lam_p (xlambda M + S) + M) v = h
+ * along z per parity (solV), u = By V Bz^T.  Neumann at xlambda = 0: zero mean.  m <= 288.     */
+int duct_setup_shen_(const char *bound, const int *m, const double *Ay, const double *Az, const double *By,
+                     const double *Bz, const double *md, const double *mu, const double *sd, const double *lam,
+                     size_t bound_len);
+int duct_helm3d_shen_dev_(const duct_devptr_t *rhs, const duct_devptr_t *sol, const int *m, const int *nmodes,
+                          const int *ib, const double *alp, const double *alambda, const char *bound,
+                          size_t bound_len);
+
 /* Mixed-boundary (temperature) solve, helm3d_temp (src/helm3d_temp.f90:2-44, int2d3d_temp.f90,
  * solvediag_temp.f90:2-352) on device fields.  duct_setup_temp_ is the GPU-upload tail of
  * pseudomaty_temp / pseudomatz_temp (src/pseudomat_temp.f90:2-321): bounds holds the four
@@ -297,7 +315,7 @@ int duct_prof_enable_(const int *on);
  * 1 proj_p2p_div, 2 gauge, 3 proj_p2p, 4 proj_p2c, 5 boundary, 6 bc_correct,
  * 7 transpose, 8 invdiag, 9 lp_c2p, 10 lp_p2p_div, 11 lp_p2p, 12 lp_p2c,
  * 13 resid, 14 deriv, 15 adjoint (Neumann boundary adjoint), 16 fold_c2p,
- * 17 fold_p2p_div, 18 fold_p2p, 19 fold_p2c, 20 xprod (advection products). */
+ * 17 fold_p2p_div, 18 fold_p2p, 19 fold_p2c, 20 xprod (advection products), 21 tridiag (Shen). */
 int duct_prof_read_(const int *nclass, double *ms, int *count);
 
 #ifdef __cplusplus

@@ -46,6 +46,9 @@ program use_duct
        1.0_c_double, 1.0_c_double, 0.01_c_double, 0.1_c_double, 1_c_int, 0.0_c_double), 'rk3')
   call duct_check(duct_setup_temp('dndn', m, P, Pinv, lam, DC, P, Pinv, lam, DC, 4_c_size_t), 'setup_temp')
   call duct_check(duct_helm3d_temp_dev(r(1), s(1), r(1), r(1), m, nm, 0_c_int, 1.0_c_double, 2.0_c_double), 'temp')
+  call duct_check(duct_setup_shen('d', m, P, P, P, P, lam, lam, lam, lam, 1_c_size_t), 'preshen')
+  call duct_check(duct_helm3d_shen_dev(r(1), s(1), m, nm, 0_c_int, 1.0_c_double, 2.0_c_double, 'd', 1_c_size_t), &
+       'shen')
   call duct_check(duct_shutdown(), 'shutdown')
 end program use_duct
 """
