@@ -72,8 +72,9 @@ def parse():
                          "traffic counters are scaled by")
     ap.add_argument("--c3", action="store_true",
                     help="only time the pseudo-spectral step at 129^2 x 64 modes, mgalx 192: advection3d, "
-                         "the backward-Euler projection substep, and one full RK3 time step (three substeps "
-                         "of duct_rk3_substep_dev_)")
+                         "the backward-Euler projection substep, one full RK3 time step (three substeps "
+                         "of duct_rk3_substep_dev_), and one field of the mixed-boundary and Shen-Legendre "
+                         "solves at the same size")
     return ap.parse_args()
 
 
@@ -122,6 +123,28 @@ def c3_bench(steps, warmup):
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / steps
         out[name] = {"ms": dt * 1e3,
+                     "kernels": {k: {"launches": kt.count[k] / steps, "avg_us": kt.ms[k] / kt.count[k] * 1e3}
+                                 for k in kt.ms if kt.count[k]}}
+    # the widened solves at the same 129^2 x 64-mode size: mixed-boundary (temperature) and Shen-Legendre
+    from duct_amd import shen, temperature, workloads as W
+    rhs = torch.from_numpy(W.rhs_field(workloads.CONFIGS[1], 0)).cuda()
+    sol = torch.empty_like(rhs)
+    mm, MM = rhs.shape[-1], rhs.shape[0]
+    bc = torch.zeros((MM, 2, mm), dtype=torch.float64, device="cuda")
+    temperature.setup_temp(mm, "dndn", 1.0, 1.0)
+    shen.setup_shen(mm, "d")
+    for name, fn in (("helm3d_temp dndn", lambda: temperature.helm3d_temp(rhs, sol, bc, bc, 1000.0)),
+                     ("helm3d_shen d", lambda: shen.helm3d_shen(rhs, sol, "d", 1000.0))):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        with duct_amd.KernelTimer() as kt:
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / steps
+        out[name] = {"ms": dt * 1e3, "mode_solves_per_s": MM / dt,
                      "kernels": {k: {"launches": kt.count[k] / steps, "avg_us": kt.ms[k] / kt.count[k] * 1e3}
                                  for k in kt.ms if kt.count[k]}}
     lines, X, M = cfg.m * cfg.m, 192, cfg.nmodes

@@ -250,6 +250,18 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
                 }
         }
     } else {
+        // divide epilogue: the row eigenvalues are loaded once, before any store
+        double lrow[TM][4];
+        if constexpr (MODE == PROJ_P2P_DIV) {
+            const double* lr = a.lam2 ? a.lam2 : a.lam;
+#pragma unroll
+            for (int mt = 0; mt < TM; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = rbase + mt * 16 + 4 * r;
+                    lrow[mt][r] = row < n ? lr[row] : 0.0;
+                }
+        }
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
             long plane;
@@ -269,13 +281,14 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
             }
             if (!ok) continue;
             double* Y = a.Y + (size_t)plane * NP * NP + j;
-            double xlam = 0.0;
+            double xlam = 0.0, lj = 0.0;
             bool gauge_plane = false;
             if constexpr (MODE == PROJ_P2P_DIV) {
                 const long fm = plane >> 1;
                 const int f = (int)(fm / g.nmodes), mode = (int)(fm % g.nmodes);
                 xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
                 gauge_plane = a.gauge >= 0 && fabs(xlam) <= 1.0e-10;
+                lj = j < n ? a.lam[j] : 0.0;
             }
 #pragma unroll
             for (int mt = 0; mt < TM; ++mt)
@@ -285,7 +298,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
                     double v = acc[mt][nt][r];
                     if constexpr (MODE == PROJ_P2P_DIV) {
                         if (j < n && row < n && !(gauge_plane && j == a.gauge && row == a.gauge))
-                            v = v / (a.lam[j] + (a.lam2 ? a.lam2 : a.lam)[row] - xlam);
+                            v = v / (lj + lrow[mt][r] - xlam);
                         else
                             v = 0.0;
                     }
