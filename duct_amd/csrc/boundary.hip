@@ -232,19 +232,43 @@ __global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
     const double det = DC00 * DCmm - DC0m * DCm0;
     d2v f1 = {0, 0}, f2 = {0, 0}, f3 = {0, 0}, f4 = {0, 0};
 
+    // Every edge value is computed from the interior first and written only at the end: a
+    // store into `out` between the interior loads would keep the later loads from issuing early.
+    // The two DC rows every sum uses are staged in LDS once, and each thread issues all the
+    // loads of one pass before it accumulates them.
+    __shared__ d2v edge_y[2][kMaxM];          // U(1, j), U(m, j)
+    __shared__ d2v edge_z[2][kMaxM];          // U(i, 1), U(i, m)
+    __shared__ double dc0[kMaxM], dcm[kMaxM]; // DC(1, :), DC(m, :)
+    for (int t = tid; t < m; t += kAuxThreads) {
+        dc0[t] = DC[(size_t)t * m];
+        dcm[t] = DC[(m - 1) + (size_t)t * m];
+    }
+    __syncthreads();
+
     // z = 1 and z = m lines: sums along z for every interior y.  Four threads
     // per y split the z range; lanes of a wave take consecutive y (coalesced).
+    constexpr int kBatch = 8;
     __shared__ d2v part[4][256][2];
     const int ty = tid & 255, zc = tid >> 8;
     for (int yb = 0; yb < m - 2; yb += 256) {
         const int i = 1 + yb + ty;
         d2v sb = {0.0, 0.0}, stp = {0.0, 0.0};
         if (i < m - 1) {
-#pragma unroll 4
-            for (int j = 1 + zc; j < m - 1; j += 4) {
-                const d2v u = out[(size_t)i + (size_t)j * m];
-                sb += DC[(size_t)j * m] * u;
-                stp += DC[(m - 1) + (size_t)j * m] * u;
+            for (int j0 = 1 + zc; j0 < m - 1; j0 += 4 * kBatch) {
+                d2v u[kBatch];
+#pragma unroll
+                for (int b = 0; b < kBatch; ++b) {
+                    const int j = j0 + 4 * b;
+                    u[b] = j < m - 1 ? out[(size_t)i + (size_t)j * m] : d2v{0.0, 0.0};
+                }
+#pragma unroll
+                for (int b = 0; b < kBatch; ++b) {
+                    const int j = j0 + 4 * b;
+                    if (j < m - 1) {
+                        sb += dc0[j] * u[b];
+                        stp += dcm[j] * u[b];
+                    }
+                }
             }
         }
         part[zc][ty][0] = sb;
@@ -255,9 +279,9 @@ __global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
             stp = (part[0][ty][1] + part[1][ty][1]) + (part[2][ty][1] + part[3][ty][1]);
             const d2v vb = (-DCmm * sb + DC0m * stp) / det;
             const d2v vt = (DCm0 * sb - DC00 * stp) / det;
-            out[(size_t)i] = vb;
-            out[(size_t)i + (size_t)(m - 1) * m] = vt;
-            const double d0 = DC[(size_t)i * m], dm = DC[(m - 1) + (size_t)i * m];
+            edge_z[0][i] = vb;
+            edge_z[1][i] = vt;
+            const double d0 = dc0[i], dm = dcm[i];
             f1 -= d0 * vb;
             f2 += dm * vb;
             f3 -= dm * vt;
@@ -265,27 +289,45 @@ __global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
         }
         __syncthreads();
     }
-    // y = 1 and y = m lines: sums along y (contiguous), one wave per z
+    // y = 1 and y = m lines: sums along y (contiguous), one wave per z; a lane loads all
+    // its rows of the column (at most kRows) before accumulating them
+    constexpr int kRows = (kMaxM + 63) / 64;
     for (int j = 1 + wave; j < m - 1; j += kAuxWaves) {
+        d2v u[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            const int i = 1 + lane + 64 * r;
+            u[r] = i < m - 1 ? out[(size_t)i + (size_t)j * m] : d2v{0.0, 0.0};
+        }
         d2v sl = {0.0, 0.0}, sr = {0.0, 0.0};
-        for (int i = 1 + lane; i < m - 1; i += 64) {
-            const d2v u = out[(size_t)i + (size_t)j * m];
-            sl += DC[(size_t)i * m] * u;
-            sr += DC[(m - 1) + (size_t)i * m] * u;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            const int i = 1 + lane + 64 * r;
+            if (i < m - 1) {
+                sl += dc0[i] * u[r];
+                sr += dcm[i] * u[r];
+            }
         }
         sl = wave_sum2(sl);
         sr = wave_sum2(sr);
         if (lane == 0) {
             const d2v vl = (-DCmm * sl + DC0m * sr) / det;
             const d2v vr = (DCm0 * sl - DC00 * sr) / det;
-            out[(size_t)j * m] = vl;
-            out[(m - 1) + (size_t)j * m] = vr;
-            const double d0 = DC[(size_t)j * m], dm = DC[(m - 1) + (size_t)j * m];
+            edge_y[0][j] = vl;
+            edge_y[1][j] = vr;
+            const double d0 = dc0[j], dm = dcm[j];
             f1 -= d0 * vl;
             f2 -= d0 * vr;
             f3 -= dm * vr;
             f4 += dm * vl;
         }
+    }
+    __syncthreads();
+    for (int t = 1 + tid; t < m - 1; t += kAuxThreads) {
+        out[(size_t)t] = edge_z[0][t];
+        out[(size_t)t + (size_t)(m - 1) * m] = edge_z[1][t];
+        out[(size_t)t * m] = edge_y[0][t];
+        out[(m - 1) + (size_t)t * m] = edge_y[1][t];
     }
     // corners: reduce ff over the block, U_corner = AM1 * ff
     __shared__ d2v fred[kAuxWaves][4];
