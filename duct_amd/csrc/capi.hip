@@ -1164,6 +1164,8 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
                 }
             }
             if (ok) {
+                ff.resize(ff.size() + kFoldFragSlack, 0.0);
+                fb.resize(fb.size() + kFoldFragSlack, 0.0);
                 if ((st = up(B.ffwd, ff.data(), ff.size()))) return st;
                 if ((st = up(B.fbwd, fb.data(), fb.size()))) return st;
                 if ((st = up(B.flam, fl.data(), fl.size()))) return st;

@@ -47,6 +47,10 @@ void build_frag_rect(const double* A, int R, int K, int lda, double* frag);
 
 // Even/odd (folded) projection passes: planes of four NH x NH quarters,
 // basis = [A_e; A_o] as a (2 NH) x NH fragment-layout matrix.
+// Zero doubles appended to each folded basis (fragment layout): the planar passes of k_projf
+// fetch one stage of A fragments past the end after their last stage.
+constexpr int kFoldFragSlack = 256;
+
 struct ProjFoldArgs {
     const double* Afrag;  // folded basis, fragment layout
     const double* X;      // planar input (quarters)

@@ -55,6 +55,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
     constexpr bool LDIV = MODE == PROJ_P2P_LDIV;
 
     __shared__ double Bs[2][NT * LDB];
+    __shared__ double lam_s[LDIV ? NP : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -73,14 +74,17 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
     constexpr int CPT = NT / 32;
     constexpr int TPQ = 16 / CPT;
 
-    const double* srcp = nullptr;          // planar column base (k = 0)
-    const d2v* srcc = nullptr;             // complex column base (k = 0 -> y = 1)
+    // The B loads are unconditional: lanes with no column read a valid address and store zeros.
+    // A load under a branch makes the compiler wait for every outstanding load, the next
+    // stage's HBM reads included, before the first instruction that follows the branch.
+    const double* srcp = a.X;              // planar column base (k = 0)
+    const d2v* srcc = a.g.in[0] + 1 + a.g.m;   // complex column base (k = 0 -> y = 1)
     int v_dst = 0, v_dst_im = 0, kk0 = 0;
     bool src_ok = false, zero_im = false;
 
     // LDIV: column j of plane `plane` is divided by lam_j + lam_k - xlambda on load
     int ld_j = 0;
-    double ld_xlam = 0.0;
+    double ld_xlam = 0.0, ld_lamj = 0.0;
     bool ld_gauge = false;
     if constexpr (MODE == PROJ_P2P || MODE == PROJ_P2P_DIV || LDIV) {
         const int v = tid / TPC;
@@ -95,6 +99,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
             const long fm = plane >> 1;
             const int f = (int)(fm / g.nmodes), mode = (int)(fm % g.nmodes);
             if (src_ok) ld_xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
+            if (src_ok && ld_j < n) ld_lamj = (a.lam2 ? a.lam2 : a.lam)[ld_j];
             ld_gauge = a.gauge >= 0 && fabs(ld_xlam) <= 1.0e-10 && ld_j == a.gauge;
         }
     } else if constexpr (COUT) {
@@ -127,47 +132,50 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
         }
     }
 
+    if constexpr (LDIV) {
+        for (int i = tid; i < NP; i += 256) lam_s[i] = a.lam[i];
+        __syncthreads();
+    }
+
     double breg[CPLX && CIN ? 1 : DPT];
     d2v creg[CIN ? CPT : 1];
+    bool okc[CIN ? CPT : 1];
 
     auto load_b = [&](int ks) {
         const int k0 = ks * S::KT + kk0;
         if constexpr (CIN) {
 #pragma unroll
             for (int e = 0; e < CPT; ++e) {
-                d2v v = {0.0, 0.0};
-                if (src_ok && k0 + e < n) v = srcc[k0 + e];
-                creg[e] = v;
+                okc[e] = src_ok && k0 + e < n;
+                creg[e] = srcc[okc[e] ? k0 + e : 0];
             }
         } else {
 #pragma unroll
             for (int e = 0; e < DPT; e += 2) {
-                d2v v = {0.0, 0.0};
-                if (src_ok) v = *(const d2v*)(srcp + k0 + e);
+                const d2v v = *(const d2v*)(srcp + k0 + e);
                 breg[e] = v[0];
                 breg[e + 1] = v[1];
             }
-            if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
-#pragma unroll
-                for (int e = 0; e < DPT; ++e) {
-                    const int k = k0 + e;
-                    if (ld_j < n && k < n && !(ld_gauge && k == a.gauge))
-                        breg[e] = breg[e] / ((a.lam2 ? a.lam2 : a.lam)[ld_j] + a.lam[k] - ld_xlam);
-                }
-            }
         }
     };
-    auto store_b = [&](int buf) {
+    auto store_b = [&](int buf, int ks) {
         double* B = Bs[buf];
         if constexpr (CIN) {
 #pragma unroll
             for (int e = 0; e < CPT; ++e) {
-                B[v_dst * LDB + kk0 + e] = creg[e][0];
-                B[v_dst_im * LDB + kk0 + e] = zero_im ? 0.0 : creg[e][1];
+                B[v_dst * LDB + kk0 + e] = okc[e] ? creg[e][0] : 0.0;
+                B[v_dst_im * LDB + kk0 + e] = (okc[e] && !zero_im) ? creg[e][1] : 0.0;
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < DPT; ++e) B[v_dst * LDB + kk0 + e] = breg[e];
+            for (int e = 0; e < DPT; ++e) {
+                double v = src_ok ? breg[e] : 0.0;
+                if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
+                    const int k = ks * S::KT + kk0 + e;
+                    if (ld_j < n && k < n && !(ld_gauge && k == a.gauge)) v = v / (ld_lamj + lam_s[k] - ld_xlam);
+                }
+                B[v_dst * LDB + kk0 + e] = v;
+            }
         }
     };
 
@@ -190,7 +198,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
 
     load_b(0);
     load_a(0, acur);
-    store_b(0);
+    store_b(0, 0);
     __syncthreads();
 
     for (int ks = 0; ks < NK; ++ks) {
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
             }
         }
         if (ks + 1 < NK) {
-            store_b(buf ^ 1);
+            store_b(buf ^ 1, ks + 1);
 #pragma unroll
             for (int mt = 0; mt < TM; ++mt) {
                 acur[mt][0] = anext[mt][0];

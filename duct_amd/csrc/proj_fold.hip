@@ -72,7 +72,10 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
     constexpr int TPC = 16 / DPT;
     constexpr int PPT = NT / 64;              // C2P: (column, k) pairs per thread per stage
 
-    const double* srcp[2] = {nullptr, nullptr};
+    // The B loads are unconditional (lanes with no column read a valid address and store zeros):
+    // a load under a branch makes the compiler wait for every outstanding load, the next stage's
+    // HBM reads included, before the first MFMA that follows the branch.
+    const double* srcp[2] = {a.X, a.X};
     int v_dst = 0, kk0 = 0;
     bool src_ok = false;
     // LDIV: column l (parity cp) of the undivided pass-2 output, row k of parity rp,
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
             const long q = qbase + ql;
             okq[p] = false;
             zimq[p] = false;
-            colA[p] = colB[p] = nullptr;
+            colA[p] = colB[p] = g.in[0] + 1 + m;   // a valid address for lanes with no column
             if (q < a.ncols) {
                 const long fm = q / NH;
                 const int j = (int)(q % NH);
@@ -140,6 +143,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
     }
     double breg[CIN ? 1 : 2][CIN ? 1 : DPT];
     d2v creg[PPT][4];
+    bool okc[PPT];                            // C2P: the loaded column is real (else stored as zero)
 
     auto load_b = [&](int ks) {
         if constexpr (CIN) {
@@ -147,13 +151,13 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
             for (int p = 0; p < PPT; ++p) {
                 const int i = ks * S::KT + kq[p];
                 const d2v z = {0.0, 0.0};
-                creg[p][0] = creg[p][1] = creg[p][2] = creg[p][3] = z;
-                if (okq[p] && i < he) {
-                    creg[p][0] = colA[p][i];
-                    creg[p][1] = colA[p][n - 1 - i];
-                    creg[p][2] = colB[p][i];
-                    creg[p][3] = colB[p][n - 1 - i];
-                }
+                const bool ok = okq[p] && i < he;
+                const int ii = ok ? i : 0;
+                creg[p][0] = colA[p][ii];
+                creg[p][1] = colA[p][n - 1 - ii];
+                creg[p][2] = colB[p][ii];
+                creg[p][3] = colB[p][n - 1 - ii];
+                okc[p] = ok;
             }
         } else {
             const int k0 = ks * S::KT + kk0;
@@ -161,8 +165,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
             for (int rp = 0; rp < 2; ++rp)
 #pragma unroll
                 for (int e = 0; e < DPT; e += 2) {
-                    d2v v = {0.0, 0.0};
-                    if (src_ok) v = *(const d2v*)(srcp[rp] + k0 + e);
+                    const d2v v = *(const d2v*)(srcp[rp] + k0 + e);
                     breg[rp][e] = v[0];
                     breg[rp][e + 1] = v[1];
                 }
@@ -173,7 +176,10 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
         if constexpr (CIN) {
 #pragma unroll
             for (int p = 0; p < PPT; ++p) {
-                const d2v x00 = creg[p][0], x10 = creg[p][1], x01 = creg[p][2], x11 = creg[p][3];
+                const d2v z = {0.0, 0.0};
+                const bool ok = okc[p];
+                const d2v x00 = ok ? creg[p][0] : z, x10 = ok ? creg[p][1] : z;
+                const d2v x01 = ok ? creg[p][2] : z, x11 = ok ? creg[p][3] : z;
                 const d2v s0 = x00 + x10, d0 = x00 - x10, s1 = x01 + x11, d1 = x01 - x11;
                 // (row parity, column parity): ee, eo, oe, oo, each /4
                 const d2v q00 = 0.25 * (s0 + s1), q01 = 0.25 * (s0 - s1);
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
             for (int rp = 0; rp < 2; ++rp)
 #pragma unroll
                 for (int e = 0; e < DPT; ++e) {
-                    double v = breg[rp][e];
+                    double v = src_ok ? breg[rp][e] : 0.0;
                     if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
                         const int k = st_k0 + kk0 + e;
                         const int hh = rp ? (n >> 1) : he;
@@ -260,7 +266,10 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
                 for (int nt = 0; nt < 4; ++nt)
                     acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bf[mt / TH][nt], acc[mt][nt], 0, 0, 0);
             }
-            if ((kk & 1) && ks + 1 < NK) {   // this half of the A fragments is dead: fetch the next stage's
+            // This half of the A fragments is dead: fetch the next stage's. The planar passes fetch
+            // unconditionally (after the last stage from the kFoldFragSlack padding), so the loop has
+            // no branch around a load and the compiler's wait counts stay exact.
+            if ((kk & 1) && (!(CIN || COUT) || ks + 1 < NK)) {
 #pragma unroll
                 for (int mt = 0; mt < TM; ++mt) acur[mt][kk >> 1] = Ab[mt][(size_t)((ks + 1) * 2 + (kk >> 1)) * 64];
             }
