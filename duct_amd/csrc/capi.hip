@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -924,8 +925,27 @@ int ensure_xdft(int M, int X)
     return OK;
 }
 
+// Independent derivative products (y or z, each its own field, matrix, scale and beta) in one
+// launch on the library's stream, with the derivative matrices of duct_setup_ (padded to L.dc_np).
+int cline_run(int mm, int M, std::initializer_list<ClineOp> ops)
+{
+    ClineArgs c{};
+    c.nmodes = M;
+    c.m = mm;
+    c.NP = L.dc_np;
+    for (const ClineOp& o : ops) {
+        if (c.nops == kMaxClineOps) return fail(ERR_ARG, "more than %d derivative products in one launch", kMaxClineOps);
+        c.op[c.nops++] = o;
+    }
+    Timed t(PK_DERIV);
+    HIPCHK(launch_cline_ops(c, L.stream));
+    return OK;
+}
+
 // advection3d (operators.F:9-276) of device fields: the x-transforms and the six products in
-// k_xprod, then the nine derivative terms; the caller has checked the sizes.
+// k_xprod, then the nine derivative terms in the order of operators.F:91-272 per output
+// (adv1: uu_x, uv_y, uw_z; adv2: vv_y, uv_x, vw_z; adv3: ww_z, uw_x, vw_y); the caller has
+// checked the sizes.
 int advection_run(const d2v* const in[3], d2v* const out[3], int mm, int M, int X, double alp, double aspect)
 {
     int st = ensure_xdft(M, X);
@@ -942,34 +962,27 @@ int advection_run(const d2v* const in[3], d2v* const out[3], int mm, int M, int 
         Timed t(PK_XPROD);
         HIPCHK(launch_xprod(xa, L.stream));
     }
-    // adv1 = d(uu)/dx + d(uv)/dy + d(uw)/dz, adv2 = d(vv)/dy + d(uv)/dx + d(vw)/dz,
-    // adv3 = d(ww)/dz + d(uw)/dx + d(vw)/dy (operators.F:91-272)
+    // Each output keeps that order of its three terms; the y and z terms run as two grouped
+    // launches of three independent products.
     const d2v* P[6];
     for (int p = 0; p < 6; ++p) P[p] = L.prods + p * field;
     enum { UU, VV, WW, UV, UW, VW };
-    struct Term { int out, prod; char dir; };
-    const Term terms[9] = {{0, UU, 'x'}, {0, UV, 'y'}, {0, UW, 'z'}, {1, VV, 'y'}, {1, UV, 'x'}, {1, VW, 'z'},
-                           {2, WW, 'z'}, {2, UW, 'x'}, {2, VW, 'y'}};
-    for (int t = 0; t < 9; ++t) {
-        const Term& T = terms[t];
-        const double beta = (t % 3) ? 1.0 : 0.0;
-        d2v* o = out[T.out];
+    const double* D = L.dcfrag[0];
+    const double sz = 1.0 / aspect;
+    {
         Timed tm(PK_DERIV);
-        if (T.dir == 'x') {
-            HIPCHK(launch_derivx(P[T.prod], o, mm, M, 0, alp, 1.0, beta, L.stream));
-        } else {
-            CollocArgs a{};
-            a.g.nf = 1; a.g.nmodes = M; a.g.m = mm; a.g.n = mm - 2;
-            a.g.in[0] = P[T.prod];
-            a.g.out[0] = o;
-            a.Dfrag = L.dcfrag[0];
-            a.NP = L.dc_np;
-            a.scal = T.dir == 'z' ? 1.0 / aspect : 1.0;
-            a.beta = beta;
-            HIPCHK(launch_colloc(T.dir == 'y' ? CL_DY : CL_DZ, a, L.stream));
-        }
+        HIPCHK(launch_derivx(P[UU], out[0], mm, M, 0, alp, 1.0, 0.0, L.stream));
     }
-    return OK;
+    if ((st = cline_run(mm, M, {{P[UV], out[0], D, 1.0, 1.0, 0}, {P[VV], out[1], D, 1.0, 0.0, 0},
+                                {P[WW], out[2], D, sz, 0.0, 1}})))
+        return st;
+    {
+        Timed tm(PK_DERIV);
+        HIPCHK(launch_derivx(P[UV], out[1], mm, M, 0, alp, 1.0, 1.0, L.stream));
+        HIPCHK(launch_derivx(P[UW], out[2], mm, M, 0, alp, 1.0, 1.0, L.stream));
+    }
+    return cline_run(mm, M, {{P[UW], out[0], D, sz, 1.0, 1}, {P[VW], out[1], D, sz, 1.0, 1},
+                             {P[VW], out[2], D, 1.0, 1.0, 0}});
 }
 
 }  // namespace
@@ -1777,19 +1790,8 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     d2v* U[3] = {(d2v*)*u, (d2v*)*v, (d2v*)*w};
     d2v* P = (d2v*)*p;
     d2v* ADV[3] = {(d2v*)*adv1, (d2v*)*adv2, (d2v*)*adv3};
-    auto colloc = [&](const d2v* in, d2v* out, const double* frag, char dir, double scal) -> int {
-        CollocArgs c{};
-        c.g.nf = 1; c.g.nmodes = M; c.g.m = mm; c.g.n = mm - 2;
-        c.g.in[0] = in;
-        c.g.out[0] = out;
-        c.Dfrag = frag;
-        c.NP = L.dc_np;
-        c.scal = scal;
-        c.beta = 1.0;
-        Timed t(PK_DERIV);
-        HIPCHK(launch_colloc(dir == 'y' ? CL_DY : CL_DZ, c, L.stream));
-        return OK;
-    };
+    const double* D1 = L.dcfrag[0];
+    const double* D2 = L.d2frag;
     // (ii) adv(u^{k-1}) into advnew
     const d2v* vin[3] = {U[0], U[1], U[2]};
     if ((st = advection_run(vin, advnew, mm, M, X, *alp, asp))) return st;
@@ -1819,12 +1821,15 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
         Timed t(PK_DERIV);
         HIPCHK(launch_rk3_rhs(r, L.stream));
     }
-    for (int c = 0; c < 3; ++c) {   // laplacian3d's y and z parts, scaled as in (vii)
-        if ((st = colloc(U[c], h[c], L.d2frag, 'y', s * nu * a))) return st;
-        if ((st = colloc(U[c], h[c], L.d2frag, 'z', s * nu * a / (asp * asp)))) return st;
-    }
-    if ((st = colloc(P, h[1], L.dcfrag[0], 'y', -s * ab))) return st;        // grad3d, y and z
-    if ((st = colloc(P, h[2], L.dcfrag[0], 'z', -s * ab / asp))) return st;
+    // laplacian3d's y parts, then its z parts, scaled as in (vii); then grad3d's y and z parts of p
+    const double sy2 = s * nu * a, sz2 = s * nu * a / (asp * asp);
+    if ((st = cline_run(mm, M, {{U[0], h[0], D2, sy2, 1.0, 0}, {U[1], h[1], D2, sy2, 1.0, 0},
+                                {U[2], h[2], D2, sy2, 1.0, 0}})))
+        return st;
+    if ((st = cline_run(mm, M, {{U[0], h[0], D2, sz2, 1.0, 1}, {U[1], h[1], D2, sz2, 1.0, 1},
+                                {U[2], h[2], D2, sz2, 1.0, 1}})))
+        return st;
+    if ((st = cline_run(mm, M, {{P, h[1], D1, -s * ab, 1.0, 0}, {P, h[2], D1, -s * ab / asp, 1.0, 1}}))) return st;
     // (viii) u*, v*, w*: Dirichlet Helmholtz, alambda = 1/(b nu dt)
     {
         const double helco = 1.0 / (b * nu * *dt);
@@ -1839,8 +1844,8 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
         Timed t(PK_DERIV);
         HIPCHK(launch_xderiv(U[0], h[0], (long)mm * mm, M, 0, 1, *alp, sd, 0.0, L.stream));
     }
-    if ((st = colloc(U[1], h[0], L.dcfrag[0], 'y', sd))) return st;
-    if ((st = colloc(U[2], h[0], L.dcfrag[0], 'z', sd / asp))) return st;
+    if ((st = cline_run(mm, M, {{U[1], h[0], D1, sd, 1.0, 0}}))) return st;
+    if ((st = cline_run(mm, M, {{U[2], h[0], D1, sd / asp, 1.0, 1}}))) return st;
     // (x) phi: Neumann Poisson
     {
         const duct_devptr_t rhs[1] = {(duct_devptr_t)h[0]};
@@ -1854,15 +1859,13 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
         Timed t(PK_DERIV);
         HIPCHK(launch_xderiv(phi, h[0], (long)mm * mm, M, 0, 2, *alp, sl, 0.0, L.stream));
     }
-    if ((st = colloc(phi, h[0], L.d2frag, 'y', sl))) return st;
-    if ((st = colloc(phi, h[0], L.d2frag, 'z', sl / (asp * asp)))) return st;
+    if ((st = cline_run(mm, M, {{phi, h[0], D2, sl, 1.0, 0}}))) return st;
+    if ((st = cline_run(mm, M, {{phi, h[0], D2, sl / (asp * asp), 1.0, 1}}))) return st;
     {
         Timed t(PK_DERIV);
         HIPCHK(launch_rk3_update(P, U[0], phi, h[0], (long)mm * mm, M, 0, *alp, su, L.stream));
     }
-    if ((st = colloc(phi, U[1], L.dcfrag[0], 'y', su))) return st;
-    if ((st = colloc(phi, U[2], L.dcfrag[0], 'z', su / asp))) return st;
-    return OK;
+    return cline_run(mm, M, {{phi, U[1], D1, su, 1.0, 0}, {phi, U[2], D1, su / asp, 1.0, 1}});
 }
 
 // The GPU-upload tails of pseudomaty_temp / pseudomatz_temp (pseudomat_temp.f90:2-321): the

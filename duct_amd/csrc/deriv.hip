@@ -1,16 +1,21 @@
-// Collocation derivative products on whole complex(8) planes, one direction per launch:
+// Collocation derivative products on whole complex(8) planes:
 //   y:  out(y, z) = beta out + scal sum_k D(y, k) u(k, z)      (D u, along the contiguous y)
 //   z:  out(y, z) = beta out + scal sum_k D(z, k) u(y, k)      (u D^T, along z)
-// for every plane of every field of a FieldGroup (derivyz / transyz applied along y or z,
+// for every plane of one or more fields (derivyz / transyz applied along y or z,
 // chebyshev.f90:1-540, as a dense product with the matrices of the m CGL nodes).
 //
-// Work decomposition: a workgroup owns one 16-line slab of one plane, the lines being the
-// direction that is not contracted (16 z for y, 16 y for z), and stages that slab of u,
-// all K = 16 ceil(m/16) contraction points, re and im, in LDS once. Its three waves then
-// cover the NT = ceil(m/16) output tiles along the contracted direction, TPW tiles each,
-// streaming the matrix fragments from L2. At m = 129 the padding costs (144/129)^3 = 1.4x
-// the useful MFMA work, against 2.5x for 64 x 64 output blocks, and a slab is 37 KB of LDS,
-// so four workgroups share a CU (static LDS sized for NP <= 48 TPW).
+// Work decomposition: a workgroup owns one 16-line slab of one plane of one op, the lines being
+// the direction that is not contracted (16 z for y, 16 y for z), and stages that slab of u, all
+// K = 16 ceil(m/16) contraction points, re and im, in LDS once. Its three waves then cover the
+// NT = ceil(m/16) output tiles along the contracted direction, TPW tiles each, streaming the
+// matrix fragments from L2. At m = 129 the padding costs (144/129)^3 = 1.4x the useful MFMA work,
+// against 2.5x for 64 x 64 output blocks, and a slab is 37 KB of LDS, so four workgroups share a
+// CU (static LDS sized for NP <= 48 TPW).
+//
+// One launch runs a list of independent ops (its own field, matrix, direction, scale and beta
+// each): a single field's planes give about two waves per SIMD at m = 129, too few to hide the
+// L2 latency of the fragments or to balance the CUs, so callers group the products that do not
+// depend on each other.
 #include "common.h"
 #include "proj_lp.h"
 
@@ -22,50 +27,59 @@ constexpr int kThreads = 64 * kWaves;
 }  // namespace
 
 template <bool DIRZ, int TPW>
-__global__ __launch_bounds__(kThreads, 2) void k_cline(CollocArgs a)
+__device__ __forceinline__ void cline_slab(const d2v* __restrict__ U, d2v* O, const double* Dfrag, int m, int NP,
+                                           int lb, double scal, double beta, double* slab)
 {
-    __shared__ double slab[2 * 16 * (16 * kWaves * TPW + 1)];   // [part][line 0..15][k 0..K-1], stride LDK
-    const FieldGroup& g = a.g;
-    const int m = g.m, NT = a.NP / 16, K = a.NP, LDK = K + 1;
-    const long plane = blockIdx.x / NT;
-    const int lb = (int)(blockIdx.x % NT);        // 16-line block along the uncontracted direction
-    const int f = (int)(plane / g.nmodes), mode = (int)(plane % g.nmodes);
-    const d2v* U = g.in[f] + (size_t)mode * m * m;
-    d2v* O = g.out[f] + (size_t)mode * m * m;
+    const int NT = NP / 16, K = NP, LDK = K + 1;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar branches only
     const int l16 = lane & 15, l4 = lane >> 4;
     const int L0 = lb * 16;
 
     // stage the slab: y: u(k, z = L0 + l) is contiguous in k;  z: u(y = L0 + l, k) has y contiguous.
-    // All loads of a thread are issued before its LDS stores, so their latencies overlap.
+    // All loads of a thread are issued before its LDS stores, so their latencies overlap; they are
+    // unconditional (points outside the plane read u(0, 0) and are stored as zeros).
     double* Sre = slab;
     double* Sim = slab + 16 * LDK;
     constexpr int IT = (16 * 16 * kWaves * TPW + kThreads - 1) / kThreads;
+    const float invK = 1.0f / (float)K;       // e / K below is exact: e < 2^13, K <= 288
     d2v v[IT];
+    bool ok[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int e = tid + it * kThreads;
-        v[it] = d2v{0.0, 0.0};
-        if (e < 16 * K) {
-            const int l = DIRZ ? e % 16 : e / K, k = DIRZ ? e / 16 : e % K;
-            if (L0 + l < m && k < m)
-                v[it] = DIRZ ? U[(size_t)(L0 + l) + (size_t)k * m] : U[(size_t)k + (size_t)(L0 + l) * m];
+        int l, k;
+        if constexpr (DIRZ) {
+            l = e & 15;
+            k = e >> 4;
+        } else {
+            l = (int)(((float)e + 0.5f) * invK);
+            k = e - l * K;
         }
+        ok[it] = e < 16 * K && L0 + l < m && k < m;
+        const size_t idx = DIRZ ? (size_t)(L0 + l) + (size_t)k * m : (size_t)k + (size_t)(L0 + l) * m;
+        v[it] = U[ok[it] ? idx : 0];
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int e = tid + it * kThreads;
         if (e < 16 * K) {
-            const int l = DIRZ ? e % 16 : e / K, k = DIRZ ? e / 16 : e % K;
-            Sre[l * LDK + k] = v[it][0];
-            Sim[l * LDK + k] = v[it][1];
+            int l, k;
+            if constexpr (DIRZ) {
+                l = e & 15;
+                k = e >> 4;
+            } else {
+                l = (int)(((float)e + 0.5f) * invK);
+                k = e - l * K;
+            }
+            Sre[l * LDK + k] = ok[it] ? v[it][0] : 0.0;
+            Sim[l * LDK + k] = ok[it] ? v[it][1] : 0.0;
         }
     }
     __syncthreads();
 
     // tiles of this wave: t = wave + kWaves s
-    const d2v* Df = (const d2v*)a.Dfrag;
+    const d2v* Df = (const d2v*)Dfrag;
     d4v acc[TPW][2];
 #pragma unroll
     for (int s = 0; s < TPW; ++s) acc[s][0] = acc[s][1] = d4v{0.0, 0.0, 0.0, 0.0};
@@ -111,7 +125,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_cline(CollocArgs a)
     // The accumulated outputs are all loaded before the first store: out and u may be the same
     // allocation for the compiler, which would otherwise serialise every load behind a store.
     d2v ov[TPW][4];
-    if (a.beta != 0.0) {
+    if (beta != 0.0) {
 #pragma unroll
         for (int s = 0; s < TPW; ++s) {
             const int t = wave + kWaves * s;
@@ -134,26 +148,52 @@ __global__ __launch_bounds__(kThreads, 2) void k_cline(CollocArgs a)
             const int z = DIRZ ? 16 * t + i : L0 + i;
             const int y = DIRZ ? L0 + l16 : 16 * t + l16;
             if (z < m && y < m) {
-                d2v v = {a.scal * acc[s][0][e], a.scal * acc[s][1][e]};
-                if (a.beta != 0.0) v += a.beta * ov[s][e];
-                O[(size_t)y + (size_t)z * m] = v;
+                d2v r = {scal * acc[s][0][e], scal * acc[s][1][e]};
+                if (beta != 0.0) r += beta * ov[s][e];
+                O[(size_t)y + (size_t)z * m] = r;
             }
         }
     }
 }
 
-template <bool DIRZ>
-static hipError_t launch_dir(const CollocArgs& a, hipStream_t st)
+template <int TPW>
+__global__ __launch_bounds__(kThreads, 2) void k_cline(ClineArgs a)
 {
+    __shared__ double slab[2 * 16 * (16 * kWaves * TPW + 1)];   // [part][line 0..15][k 0..K-1], stride LDK
+    const int NT = a.NP / 16, m = a.m;
+    const long per_op = (long)a.nmodes * NT;
+    const int o = (int)(blockIdx.x / per_op);
+    const long r = blockIdx.x % per_op;
+    const int mode = (int)(r / NT), lb = (int)(r % NT);
+    const ClineOp& op = a.op[o];
+    const d2v* U = op.in + (size_t)mode * m * m;
+    d2v* O = op.out + (size_t)mode * m * m;
+    if (op.dirz)
+        cline_slab<true, TPW>(U, O, op.Dfrag, m, a.NP, lb, op.scal, op.beta, slab);
+    else
+        cline_slab<false, TPW>(U, O, op.Dfrag, m, a.NP, lb, op.scal, op.beta, slab);
+}
+
+hipError_t launch_cline_ops(const ClineArgs& a, hipStream_t st)
+{
+    // the matrix fragments cover NP = 16 ceil(m/16) >= m; at most 6 tiles per wave (m <= 288)
+    if (a.NP % 16 || a.NP < a.m || a.NP > 16 * kWaves * 6 || a.nops < 0 || a.nops > kMaxClineOps)
+        return hipErrorInvalidValue;
+    for (int i = 0; i < a.nops; ++i) {
+        if (!a.op[i].in || !a.op[i].out || !a.op[i].Dfrag) return hipErrorInvalidValue;
+        for (int j = 0; j < a.nops; ++j)   // every output is written by one op and read by no other
+            if ((j != i && a.op[j].out == a.op[i].out) || a.op[j].in == a.op[i].out) return hipErrorInvalidValue;
+    }
+    if (a.nops == 0 || a.nmodes <= 0) return hipSuccess;
     const int NT = a.NP / 16;
-    const long blocks = (long)a.g.nf * a.g.nmodes * NT;
+    const long blocks = (long)a.nops * a.nmodes * NT;
     switch ((NT + kWaves - 1) / kWaves) {
-        case 1: hipLaunchKernelGGL((k_cline<DIRZ, 1>), dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_cline<DIRZ, 2>), dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
-        case 3: hipLaunchKernelGGL((k_cline<DIRZ, 3>), dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_cline<DIRZ, 4>), dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
-        case 5: hipLaunchKernelGGL((k_cline<DIRZ, 5>), dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
-        case 6: hipLaunchKernelGGL((k_cline<DIRZ, 6>), dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+        case 1: hipLaunchKernelGGL(k_cline<1>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+        case 2: hipLaunchKernelGGL(k_cline<2>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+        case 3: hipLaunchKernelGGL(k_cline<3>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+        case 4: hipLaunchKernelGGL(k_cline<4>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+        case 5: hipLaunchKernelGGL(k_cline<5>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+        case 6: hipLaunchKernelGGL(k_cline<6>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -161,10 +201,14 @@ static hipError_t launch_dir(const CollocArgs& a, hipStream_t st)
 
 hipError_t launch_cline(int dirz, const CollocArgs& a, hipStream_t st)
 {
-    // the matrix fragments cover NP = 16 ceil(m/16) >= m; at most 6 tiles per wave (m <= 288)
-    if (a.NP % 16 || a.NP < a.g.m || a.NP > 16 * kWaves * 6 || a.g.nf < 1) return hipErrorInvalidValue;
-    if ((long)a.g.nf * a.g.nmodes <= 0) return hipSuccess;
-    return dirz ? launch_dir<true>(a, st) : launch_dir<false>(a, st);
+    if (a.g.nf < 1 || a.g.nf > kMaxClineOps) return hipErrorInvalidValue;
+    ClineArgs c{};
+    c.nops = a.g.nf;
+    c.nmodes = a.g.nmodes;
+    c.m = a.g.m;
+    c.NP = a.NP;
+    for (int f = 0; f < a.g.nf; ++f) c.op[f] = ClineOp{a.g.in[f], a.g.out[f], a.Dfrag, a.scal, a.beta, dirz};
+    return launch_cline_ops(c, st);
 }
 
 }  // namespace duct

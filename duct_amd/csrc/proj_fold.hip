@@ -150,7 +150,6 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
 #pragma unroll
             for (int p = 0; p < PPT; ++p) {
                 const int i = ks * S::KT + kq[p];
-                const d2v z = {0.0, 0.0};
                 const bool ok = okq[p] && i < he;
                 const int ii = ok ? i : 0;
                 creg[p][0] = colA[p][ii];

@@ -54,6 +54,23 @@ hipError_t launch_resid(const ResidArgs& a, hipStream_t st);
 // Whole-plane derivative product along y (dirz = 0) or z (dirz = 1) for every plane of a.g:
 // out = beta out + scal (D u | u D^T), one 16-line slab per workgroup (deriv.hip); NP <= 288.
 hipError_t launch_cline(int dirz, const CollocArgs& a, hipStream_t st);
+
+// Several independent derivative products in one launch (deriv.hip): op k computes
+//   out_k = beta_k out_k + scal_k (D_k u_k | u_k D_k^T)     (dirz_k = 0 | 1)
+// for every plane of u_k. The outputs must be distinct and no output may be another op's input.
+constexpr int kMaxClineOps = 8;
+struct ClineOp {
+    const d2v* in;
+    d2v* out;
+    const double* Dfrag;          // the matrix in the fragment layout of build_frag, NP padded
+    double scal, beta;
+    int dirz;
+};
+struct ClineArgs {
+    ClineOp op[kMaxClineOps];
+    int nops, nmodes, m, NP;
+};
+hipError_t launch_cline_ops(const ClineArgs& a, hipStream_t st);
 hipError_t launch_derivx(const d2v* in, d2v* out, int m, int nmodes, int mode0, double alp, double scal, double beta,
                          hipStream_t st);
 
