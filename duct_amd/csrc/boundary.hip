@@ -10,11 +10,35 @@
 
 namespace duct {
 
+// Sum over the 64 lanes of a wave (all lanes active), the same value returned to every lane:
+// DPP moves within rows of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror)
+// leave each row's sum in all its lanes; the four row sums are read from lanes 0, 16, 32, 48.
+// These are VALU moves, where a shuffle (__shfl_xor) goes through the LDS crossbar and a chain
+// of six of them per value made the reductions the bound of the Neumann edge kernels.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_row<0xB1>(v);    // quad_perm [1,0,3,2]: pairs
+    v += dpp_row<0x4E>(v);    // quad_perm [2,3,0,1]: quads
+    v += dpp_row<0x141>(v);   // row_half_mirror: 8 lanes
+    v += dpp_row<0x140>(v);   // row_mirror: 16 lanes
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 __device__ __forceinline__ d2v wave_sum2(d2v v)
@@ -30,34 +54,64 @@ __device__ __forceinline__ d2v wave_sum2(d2v v)
 constexpr int kAuxThreads = 1024;
 constexpr int kAuxWaves = kAuxThreads / 64;
 
+// The sums of the gauge kernels: lanes run down the contiguous l (LI steps of 64), the waves split
+// k, and each wave takes its k values in chunks of KCH whose loads are all issued before any is
+// summed; a loop that loaded and summed one term at a time left every load's latency exposed.
+template <int LI>
+struct GaugeShape {
+    static constexpr int KCH = LI <= 2 ? 8 : (16 / LI >= 2 ? 16 / LI : 2);
+};
+
+template <int LI, bool F32>
 __global__ __launch_bounds__(kAuxThreads) void k_gauge(GaugeArgs a, GaugeList list)
 {
+    constexpr int KCH = GaugeShape<LI>::KCH;
     const int plane = list.plane[blockIdx.x];
     const int NP = a.NP, n = a.g.n, io = a.gauge;
     const size_t base = (size_t)plane * NP * NP;
     const double* Gd = (const double*)a.G + base;
     const float* Gf = (const float*)a.G + base;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double part = 0.0;
-    if (a.lam) {   // undivided G: divide each term as pass 2 would have (the (a,a) entry counts 0)
+    const bool divide = a.lam != nullptr;   // undivided G: divide each term as pass 2 would have
+    double xlam = 0.0;
+    if (divide) {
         const int fm = plane >> 1;
-        const double xlam = xlambda_of(a.g.alambda[fm / a.g.nmodes], a.g.alp, a.g.mode0 + fm % a.g.nmodes);
-        for (int k = wave; k < n; k += kAuxWaves) {
-            const double pk = a.prow[k];
-            const size_t col = (size_t)k * NP;
-            for (int l = lane; l < n; l += 64)
-                if (!(l == io && k == io)) part += a.prow[l] * (Gd[col + l] / (a.lam[l] + a.lam[k] - xlam)) * pk;
+        xlam = xlambda_of(a.g.alambda[fm / a.g.nmodes], a.g.alp, a.g.mode0 + fm % a.g.nmodes);
+    }
+    double pl[LI], ll[LI];
+#pragma unroll
+    for (int h = 0; h < LI; ++h) {
+        const int l = lane + 64 * h, lc = l < NP ? l : 0;
+        pl[h] = a.prow[lc];
+        ll[h] = divide ? a.lam[lc] : 0.0;
+    }
+    double part = 0.0;
+    for (int k0 = wave; k0 < n; k0 += kAuxWaves * KCH) {
+        double gv[KCH][LI], pk[KCH], lk[KCH];
+#pragma unroll
+        for (int t = 0; t < KCH; ++t) {
+            const int k = k0 + kAuxWaves * t, kc = k < n ? k : 0;
+            pk[t] = a.prow[kc];
+            lk[t] = divide ? a.lam[kc] : 0.0;
+#pragma unroll
+            for (int h = 0; h < LI; ++h) {
+                const int l = lane + 64 * h, lc = l < NP ? l : 0;
+                gv[t][h] = F32 ? (double)Gf[(size_t)kc * NP + lc] : Gd[(size_t)kc * NP + lc];
+            }
         }
-    } else {
-        for (int k = wave; k < n; k += kAuxWaves) {
-            const double pk = a.prow[k];
-            const size_t col = (size_t)k * NP;
-            if (a.f32) {
-#pragma unroll 4
-                for (int l = lane; l < n; l += 64) part += a.prow[l] * (double)Gf[col + l] * pk;
-            } else {
-#pragma unroll 4
-                for (int l = lane; l < n; l += 64) part += a.prow[l] * Gd[col + l] * pk;
+#pragma unroll
+        for (int t = 0; t < KCH; ++t) {
+            const int k = k0 + kAuxWaves * t;
+#pragma unroll
+            for (int h = 0; h < LI; ++h) {
+                const int l = lane + 64 * h;
+                if (k < n && l < n) {
+                    if (divide) {   // the (a,a) entry counts 0
+                        if (!(l == io && k == io)) part += pl[h] * (gv[t][h] / (ll[h] + lk[t] - xlam)) * pk[t];
+                    } else {
+                        part += pl[h] * gv[t][h] * pk[t];
+                    }
+                }
             }
         }
     }
@@ -76,15 +130,28 @@ __global__ __launch_bounds__(kAuxThreads) void k_gauge(GaugeArgs a, GaugeList li
             target -= u[part];
         }
         const double v = (target - valore) / (a.pdiag * a.pdiag);
-        if (a.f32) ((float*)a.G)[base + io + (size_t)io * NP] = (float)v;
+        if (F32) ((float*)a.G)[base + io + (size_t)io * NP] = (float)v;
         else ((double*)a.G)[base + io + (size_t)io * NP] = v;
     }
+}
+
+template <int LI>
+static void launch_gauge_li(const GaugeArgs& a, const GaugeList& list, hipStream_t st)
+{
+    if (a.f32) hipLaunchKernelGGL((k_gauge<LI, true>), dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
+    else hipLaunchKernelGGL((k_gauge<LI, false>), dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
 }
 
 hipError_t launch_gauge(const GaugeArgs& a, const GaugeList& list, hipStream_t st)
 {
     if (list.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gauge, dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
+    switch (a.NP <= 64 ? 1 : a.NP / 64) {   // lanes cover the NP (<= 512) rows of a column
+        case 1: launch_gauge_li<1>(a, list, st); break;
+        case 2: launch_gauge_li<2>(a, list, st); break;
+        case 4: launch_gauge_li<4>(a, list, st); break;
+        case 8: launch_gauge_li<8>(a, list, st); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -93,10 +160,13 @@ hipError_t launch_gauge(const GaugeArgs& a, const GaugeList& list, hipStream_t s
 // quarters of s_r s_c (P_r G_rc P_c^T)(a', a'), a' the folded index of a and
 // s_o = -1 when a lies in the mirrored half; only the ee quarter holds the
 // null mode, so its (io, io) entry is set as in the unfolded gauge.
+template <int LI>   // LI = NH / 64
 __global__ __launch_bounds__(kAuxThreads) void k_gauge_fold(GaugeFoldArgs a, GaugeList list)
 {
+    constexpr int KCH = GaugeShape<LI>::KCH;
+    constexpr int NH = 64 * LI;
     const int plane = list.plane[blockIdx.x];
-    const int NH = a.NH, n = a.g.n;
+    const int n = a.g.n;
     const int hh[2] = {(n + 1) / 2, n / 2};
     double* Gp = a.G + (size_t)plane * 4 * NH * NH;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -110,11 +180,32 @@ __global__ __launch_bounds__(kAuxThreads) void k_gauge_fold(GaugeFoldArgs a, Gau
         const double* pc = a.prow + c * NH;
         const double* lr = a.lam + r * NH;
         const double* lc = a.lam + c * NH;
-        for (int k = wave; k < hh[c]; k += kAuxWaves) {
-            const double pk = pc[k];
-            for (int l = lane; l < hh[r]; l += 64)
-                if (!(q == 0 && l == a.io && k == a.io))
-                    part += pr[l] * (G[(size_t)k * NH + l] / (lr[l] + lc[k] - xlam)) * pk;
+        double pl[LI], ll[LI];
+#pragma unroll
+        for (int h = 0; h < LI; ++h) {
+            pl[h] = pr[lane + 64 * h];
+            ll[h] = lr[lane + 64 * h];
+        }
+        for (int k0 = wave; k0 < hh[c]; k0 += kAuxWaves * KCH) {
+            double gv[KCH][LI], pk[KCH], lk[KCH];
+#pragma unroll
+            for (int t = 0; t < KCH; ++t) {
+                const int k = k0 + kAuxWaves * t, kc = k < hh[c] ? k : 0;
+                pk[t] = pc[kc];
+                lk[t] = lc[kc];
+#pragma unroll
+                for (int h = 0; h < LI; ++h) gv[t][h] = G[(size_t)kc * NH + lane + 64 * h];
+            }
+#pragma unroll
+            for (int t = 0; t < KCH; ++t) {
+                const int k = k0 + kAuxWaves * t;
+#pragma unroll
+                for (int h = 0; h < LI; ++h) {
+                    const int l = lane + 64 * h;
+                    if (k < hh[c] && l < hh[r] && !(q == 0 && l == a.io && k == a.io))
+                        part += pl[h] * (gv[t][h] / (ll[h] + lk[t] - xlam)) * pk[t];
+                }
+            }
         }
     }
     __shared__ double red[kAuxWaves];
@@ -131,7 +222,13 @@ __global__ __launch_bounds__(kAuxThreads) void k_gauge_fold(GaugeFoldArgs a, Gau
 hipError_t launch_gauge_fold(const GaugeFoldArgs& a, const GaugeList& list, hipStream_t st)
 {
     if (list.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gauge_fold, dim3(list.count), dim3(kAuxThreads), 0, st, a, list);
+    const dim3 grid(list.count), block(kAuxThreads);
+    switch (a.NH) {
+        case 64: hipLaunchKernelGGL(k_gauge_fold<1>, grid, block, 0, st, a, list); break;
+        case 128: hipLaunchKernelGGL(k_gauge_fold<2>, grid, block, 0, st, a, list); break;
+        case 256: hipLaunchKernelGGL(k_gauge_fold<4>, grid, block, 0, st, a, list); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -225,112 +322,163 @@ __global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
         }
         return;
     }
+}
 
+// Neumann edges and corners (solvediag.f90:251-322) in two launches:
+//  k_bnd_cols: kBndSplit workgroups per plane, each owning a range of interior columns j. They read
+//    the plane's interior once between them. Each computes the edges U(1, j), U(m, j) of its columns
+//    from the sums along y (sl_j = sum_i DC(1,i) U(i,j), sr_j with DC(m,:)), and its share of the sums
+//    along z of every interior row (sb_i = sum_j DC(1,j) U(i,j), st_i), which go to a.part.
+//  k_bnd_rows: one workgroup per plane adds the shares, writes the edges U(i, 1), U(i, m), and
+//    sets the corners from AM1 and the sums ff over all four edges.
+// A wave owns whole columns (its lanes run down all the rows), so each column needs one reduction
+// across the lanes; the row shares of the four waves are added in LDS at the end. The loads of
+// kBndCols columns are issued before any of them is summed.
+namespace {
+constexpr int kBndThreads = 256;
+constexpr int kBndWaves = kBndThreads / 64;
+constexpr int kBndColsMax = (kMaxM + kBndSplit - 1) / kBndSplit;   // columns per workgroup
+constexpr int kBndCols = 2;                                        // columns per wave per batch of loads
+}  // namespace
+
+template <int RW>   // rows per lane: ceil((m - 2) / 64)
+__global__ __launch_bounds__(kBndThreads) void k_bnd_cols(BoundaryArgs a)
+{
+    const FieldGroup& g = a.g;
+    const int fm = blockIdx.x / kBndSplit, sp = blockIdx.x % kBndSplit;
+    const int f = fm / g.nmodes, mode = fm % g.nmodes;
+    const int m = g.m, nc = m - 2;
+    d2v* out = g.out[f] + (size_t)mode * m * m;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int J = (nc + kBndSplit - 1) / kBndSplit;
+    const int j0 = 1 + sp * J, j1 = min(m - 1, j0 + J);
+
+    __shared__ double dc0[kMaxM], dcm[kMaxM];   // DC(1, :), DC(m, :)
+    __shared__ d2v edge[2][kBndColsMax];        // U(1, j), U(m, j) of this workgroup's columns
+    __shared__ d2v racc[2][kMaxM];              // the waves' row shares, added in turn
     const double* DC = a.DC;
-    const double DC00 = DC[0], DCmm = DC[(m - 1) + (size_t)(m - 1) * m];
-    const double DC0m = DC[(size_t)(m - 1) * m], DCm0 = DC[m - 1];
-    const double det = DC00 * DCmm - DC0m * DCm0;
-    d2v f1 = {0, 0}, f2 = {0, 0}, f3 = {0, 0}, f4 = {0, 0};
-
-    // Every edge value is computed from the interior first and written only at the end: a
-    // store into `out` between the interior loads would keep the later loads from issuing early.
-    // The two DC rows every sum uses are staged in LDS once, and each thread issues all the
-    // loads of one pass before it accumulates them.
-    __shared__ d2v edge_y[2][kMaxM];          // U(1, j), U(m, j)
-    __shared__ d2v edge_z[2][kMaxM];          // U(i, 1), U(i, m)
-    __shared__ double dc0[kMaxM], dcm[kMaxM]; // DC(1, :), DC(m, :)
-    for (int t = tid; t < m; t += kAuxThreads) {
+    for (int t = tid; t < m; t += kBndThreads) {
         dc0[t] = DC[(size_t)t * m];
         dcm[t] = DC[(m - 1) + (size_t)t * m];
     }
     __syncthreads();
+    const double DC00 = DC[0], DCmm = DC[(m - 1) + (size_t)(m - 1) * m];
+    const double DC0m = DC[(size_t)(m - 1) * m], DCm0 = DC[m - 1];
+    const double det = DC00 * DCmm - DC0m * DCm0;
 
-    // z = 1 and z = m lines: sums along z for every interior y.  Four threads
-    // per y split the z range; lanes of a wave take consecutive y (coalesced).
-    constexpr int kBatch = 8;
-    __shared__ d2v part[4][256][2];
-    const int ty = tid & 255, zc = tid >> 8;
-    for (int yb = 0; yb < m - 2; yb += 256) {
-        const int i = 1 + yb + ty;
-        d2v sb = {0.0, 0.0}, stp = {0.0, 0.0};
-        if (i < m - 1) {
-            for (int j0 = 1 + zc; j0 < m - 1; j0 += 4 * kBatch) {
-                d2v u[kBatch];
+    // rows of this lane: i = 1 + lane + 64 r
+    double w0[RW], wm[RW];
 #pragma unroll
-                for (int b = 0; b < kBatch; ++b) {
-                    const int j = j0 + 4 * b;
-                    u[b] = j < m - 1 ? out[(size_t)i + (size_t)j * m] : d2v{0.0, 0.0};
-                }
+    for (int r = 0; r < RW; ++r) {
+        const int i = 1 + lane + 64 * r;
+        w0[r] = i < m - 1 ? dc0[i] : 0.0;   // zero weights past the interior
+        wm[r] = i < m - 1 ? dcm[i] : 0.0;
+    }
+    d2v rb[RW], rt[RW];
 #pragma unroll
-                for (int b = 0; b < kBatch; ++b) {
-                    const int j = j0 + 4 * b;
-                    if (j < m - 1) {
-                        sb += dc0[j] * u[b];
-                        stp += dcm[j] * u[b];
-                    }
+    for (int r = 0; r < RW; ++r) rb[r] = rt[r] = d2v{0.0, 0.0};
+    for (int jb = j0 + wave * kBndCols; jb < j1; jb += kBndWaves * kBndCols) {
+        d2v u[kBndCols][RW];
+#pragma unroll
+        for (int c = 0; c < kBndCols; ++c) {
+            const int j = jb + c < j1 ? jb + c : j0;
+#pragma unroll
+            for (int r = 0; r < RW; ++r) {
+                const int i = 1 + lane + 64 * r;
+                u[c][r] = i < m - 1 ? out[(size_t)i + (size_t)j * m] : d2v{0.0, 0.0};
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kBndCols; ++c) {
+            const int j = jb + c;
+            if (j >= j1) break;   // uniform over the wave
+            const double d0j = dc0[j], dmj = dcm[j];
+            d2v sl = {0.0, 0.0}, sr = {0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < RW; ++r) {
+                sl += w0[r] * u[c][r];
+                sr += wm[r] * u[c][r];
+                rb[r] += d0j * u[c][r];
+                rt[r] += dmj * u[c][r];
+            }
+            sl = wave_sum2(sl);
+            sr = wave_sum2(sr);
+            if (lane == 0) {
+                edge[0][j - j0] = (-DCmm * sl + DC0m * sr) / det;
+                edge[1][j - j0] = (DCm0 * sl - DC00 * sr) / det;
+            }
+        }
+    }
+    // the row shares of the four waves, added in LDS in wave order
+    for (int w = 0; w < kBndWaves; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int r = 0; r < RW; ++r) {
+                const int i = 1 + lane + 64 * r;
+                if (i < m - 1) {
+                    racc[0][i] = w ? racc[0][i] + rb[r] : rb[r];
+                    racc[1][i] = w ? racc[1][i] + rt[r] : rt[r];
                 }
             }
         }
-        part[zc][ty][0] = sb;
-        part[zc][ty][1] = stp;
-        __syncthreads();
-        if (zc == 0 && i < m - 1) {
-            sb = (part[0][ty][0] + part[1][ty][0]) + (part[2][ty][0] + part[3][ty][0]);
-            stp = (part[0][ty][1] + part[1][ty][1]) + (part[2][ty][1] + part[3][ty][1]);
-            const d2v vb = (-DCmm * sb + DC0m * stp) / det;
-            const d2v vt = (DCm0 * sb - DC00 * stp) / det;
-            edge_z[0][i] = vb;
-            edge_z[1][i] = vt;
-            const double d0 = dc0[i], dm = dcm[i];
-            f1 -= d0 * vb;
-            f2 += dm * vb;
-            f3 -= dm * vt;
-            f4 -= d0 * vt;
-        }
         __syncthreads();
     }
-    // y = 1 and y = m lines: sums along y (contiguous), one wave per z; a lane loads all
-    // its rows of the column (at most kRows) before accumulating them
-    constexpr int kRows = (kMaxM + 63) / 64;
-    for (int j = 1 + wave; j < m - 1; j += kAuxWaves) {
-        d2v u[kRows];
-#pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-            const int i = 1 + lane + 64 * r;
-            u[r] = i < m - 1 ? out[(size_t)i + (size_t)j * m] : d2v{0.0, 0.0};
-        }
-        d2v sl = {0.0, 0.0}, sr = {0.0, 0.0};
-#pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-            const int i = 1 + lane + 64 * r;
-            if (i < m - 1) {
-                sl += dc0[i] * u[r];
-                sr += dcm[i] * u[r];
-            }
-        }
-        sl = wave_sum2(sl);
-        sr = wave_sum2(sr);
-        if (lane == 0) {
-            const d2v vl = (-DCmm * sl + DC0m * sr) / det;
-            const d2v vr = (DCm0 * sl - DC00 * sr) / det;
-            edge_y[0][j] = vl;
-            edge_y[1][j] = vr;
-            const double d0 = dc0[j], dm = dcm[j];
-            f1 -= d0 * vl;
-            f2 -= d0 * vr;
-            f3 -= dm * vr;
-            f4 += dm * vl;
-        }
+    d2v* P = a.part + (size_t)(fm * kBndSplit + sp) * 2 * m;
+    for (int i = 1 + tid; i < m - 1; i += kBndThreads) {
+        P[i] = racc[0][i];
+        P[m + i] = racc[1][i];
     }
-    __syncthreads();
-    for (int t = 1 + tid; t < m - 1; t += kAuxThreads) {
-        out[(size_t)t] = edge_z[0][t];
-        out[(size_t)t + (size_t)(m - 1) * m] = edge_z[1][t];
-        out[(size_t)t * m] = edge_y[0][t];
-        out[(m - 1) + (size_t)t * m] = edge_y[1][t];
+    for (int j = j0 + tid; j < j1; j += kBndThreads) {
+        out[(size_t)j * m] = edge[0][j - j0];
+        out[(m - 1) + (size_t)j * m] = edge[1][j - j0];
+    }
+}
+
+__global__ __launch_bounds__(kBndThreads) void k_bnd_rows(BoundaryArgs a)
+{
+    const FieldGroup& g = a.g;
+    const int fm = blockIdx.x;
+    const int f = fm / g.nmodes, mode = fm % g.nmodes;
+    const int m = g.m;
+    d2v* out = g.out[f] + (size_t)mode * m * m;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double* DC = a.DC;
+    const double DC00 = DC[0], DCmm = DC[(m - 1) + (size_t)(m - 1) * m];
+    const double DC0m = DC[(size_t)(m - 1) * m], DCm0 = DC[m - 1];
+    const double det = DC00 * DCmm - DC0m * DCm0;
+    const d2v* P = a.part + (size_t)fm * kBndSplit * 2 * m;
+    d2v f1 = {0, 0}, f2 = {0, 0}, f3 = {0, 0}, f4 = {0, 0};
+    for (int t = 1 + tid; t < m - 1; t += kBndThreads) {
+        d2v pb[kBndSplit], pt[kBndSplit], el, er;
+#pragma unroll
+        for (int s = 0; s < kBndSplit; ++s) {
+            pb[s] = P[(size_t)s * 2 * m + t];
+            pt[s] = P[(size_t)s * 2 * m + m + t];
+        }
+        el = out[(size_t)t * m];            // U(1, t), U(m, t): the edges of k_bnd_cols
+        er = out[(m - 1) + (size_t)t * m];
+        const double d0 = DC[(size_t)t * m], dm = DC[(m - 1) + (size_t)t * m];
+        d2v sb = {0.0, 0.0}, st = {0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < kBndSplit; ++s) {
+            sb += pb[s];
+            st += pt[s];
+        }
+        const d2v vb = (-DCmm * sb + DC0m * st) / det;
+        const d2v vt = (DCm0 * sb - DC00 * st) / det;
+        out[(size_t)t] = vb;                // U(t, 1)
+        out[(size_t)t + (size_t)(m - 1) * m] = vt;   // U(t, m)
+        f1 -= d0 * vb;
+        f2 += dm * vb;
+        f3 -= dm * vt;
+        f4 -= d0 * vt;
+        f1 -= d0 * el;
+        f2 -= d0 * er;
+        f3 -= dm * er;
+        f4 += dm * el;
     }
     // corners: reduce ff over the block, U_corner = AM1 * ff
-    __shared__ d2v fred[kAuxWaves][4];
+    __shared__ d2v fred[kBndWaves][4];
     f1 = wave_sum2(f1);
     f2 = wave_sum2(f2);
     f3 = wave_sum2(f3);
@@ -344,7 +492,7 @@ __global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
     __syncthreads();
     if (tid < 4) {
         d2v ff[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-        for (int w = 0; w < kAuxWaves; ++w)
+        for (int w = 0; w < kBndWaves; ++w)
             for (int c = 0; c < 4; ++c) ff[c] += fred[w][c];
         const double* A = a.AM1;  // 4x4 column-major, row `tid`
         const d2v v = A[tid] * ff[0] + A[tid + 4] * ff[1] + A[tid + 8] * ff[2] + A[tid + 12] * ff[3];
@@ -357,11 +505,27 @@ __global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
     }
 }
 
+size_t boundary_scratch_bytes(const FieldGroup& g)
+{
+    return (size_t)g.nf * g.nmodes * kBndSplit * 2 * g.m * sizeof(d2v);
+}
+
 hipError_t launch_boundary(const BoundaryArgs& a, hipStream_t st)
 {
     const int planes = a.g.nf * a.g.nmodes;
     if (planes <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_boundary, dim3(planes), dim3(kAuxThreads), 0, st, a);
+    if (a.mode != BND_NEUMANN) {
+        hipLaunchKernelGGL(k_boundary, dim3(planes), dim3(kAuxThreads), 0, st, a);
+        return hipGetLastError();
+    }
+    if (!a.part || a.g.m < 3 || a.g.m > kMaxM) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)planes * kBndSplit), block(kBndThreads);
+    const int rw = (a.g.m - 2 + 63) / 64;   // rows per lane, <= 8 (m <= kMaxM)
+    if (rw <= 1) hipLaunchKernelGGL(k_bnd_cols<1>, grid, block, 0, st, a);
+    else if (rw <= 2) hipLaunchKernelGGL(k_bnd_cols<2>, grid, block, 0, st, a);
+    else if (rw <= 4) hipLaunchKernelGGL(k_bnd_cols<4>, grid, block, 0, st, a);
+    else hipLaunchKernelGGL(k_bnd_cols<8>, grid, block, 0, st, a);
+    hipLaunchKernelGGL(k_bnd_rows, dim3(planes), dim3(kBndThreads), 0, st, a);
     return hipGetLastError();
 }
 

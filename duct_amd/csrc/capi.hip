@@ -122,6 +122,8 @@ struct Lib {
     TempDev temp;                         // mixed-boundary solve
     d2v* tfield = nullptr;                // its corrected right-hand side
     size_t tfield_bytes = 0;
+    d2v* bscr = nullptr;                  // row sums of the Neumann edge kernels
+    size_t bscr_bytes = 0;
     int dc_m = 0, dc_np = 0;
     int fold = 0;                         // 1: Dirichlet groups use the folded passes, 2: Neumann too
     double* xdft[2] = {nullptr, nullptr}; // advection x-DFTs (inverse, forward), fragment layout
@@ -234,6 +236,16 @@ int ensure_planar(size_t bytes)
     return OK;
 }
 
+// k_boundary and, for the Neumann edges, its two kernels with their scratch of row sums.
+hipError_t run_boundary(BoundaryArgs& ba)
+{
+    if (ba.mode == BND_NEUMANN) {
+        if (ensure_buf(L.bscr, L.bscr_bytes, boundary_scratch_bytes(ba.g)) != OK) return hipErrorOutOfMemory;
+        ba.part = L.bscr;
+    }
+    return launch_boundary(ba, L.stream);
+}
+
 hipEvent_t take_event()
 {
     if (!L.pool.empty()) {
@@ -336,7 +348,7 @@ int run_group_fold(const FieldGroup& g, int bidx, int bmode)
         BoundaryArgs ba{};
         ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
         ba.mode = bidx == 1 ? BND_NEUMANN : bmode;
-        HIPCHK(launch_boundary(ba, L.stream));
+        HIPCHK(run_boundary(ba));
     }
     return OK;
 }
@@ -409,7 +421,7 @@ int run_group(const FieldGroup& g, int bidx, int bmode)
         BoundaryArgs ba{};
         ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
         ba.mode = bidx == 1 ? BND_NEUMANN : bmode;
-        HIPCHK(launch_boundary(ba, L.stream));
+        HIPCHK(run_boundary(ba));
     }
     return OK;
 }
@@ -496,7 +508,7 @@ int run_group_adj(const FieldGroup& g0, int bidx)
         BoundaryArgs ba{};
         ba.g = g0; ba.DC = B.DC; ba.AM1 = B.AM1;
         ba.mode = BND_OUT_ZERO;
-        HIPCHK(launch_boundary(ba, L.stream));
+        HIPCHK(run_boundary(ba));
     }
     return OK;
 }
@@ -758,7 +770,7 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
         BoundaryArgs ba{};
         ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
         ba.mode = bidx == 1 ? BND_NEUMANN : BND_DIR_ZERO;
-        HIPCHK(launch_boundary(ba, L.stream));
+        HIPCHK(run_boundary(ba));
     }
     return OK;
 }
@@ -859,7 +871,7 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
             BoundaryArgs ba{};
             ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
             ba.mode = bidx == 1 ? BND_NEUMANN : BND_DIR_ZERO;
-            HIPCHK(launch_boundary(ba, L.stream));
+            HIPCHK(run_boundary(ba));
         }
     }
     if (nstat) {
@@ -1024,6 +1036,8 @@ int duct_shutdown_(void)
     L.sgam_bytes = 0;
     free_ptr(L.tfield);
     L.tfield_bytes = 0;
+    free_ptr(L.bscr);
+    L.bscr_bytes = 0;
     free_ptr(L.xdft[0]);
     free_ptr(L.xdft[1]);
     L.xdft_M = L.xdft_X = 0;

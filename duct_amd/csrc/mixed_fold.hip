@@ -380,6 +380,10 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long* p, double v)
 }
 }  // namespace
 
+// NSRC, the number of FP32 sources summed into u, is a template parameter so that the loads of
+// a stage are all issued before any of them is waited for: they land while the MFMAs of the
+// current stage run, and are summed (in FP64, in source order) only as they go to LDS.
+template <int NSRC>
 __global__ __launch_bounds__(256, 2) void k_resid_fold(ResidFoldArgs a, long ncplx)
 {
     const FieldGroup& g = a.g;
@@ -409,36 +413,30 @@ __global__ __launch_bounds__(256, 2) void k_resid_fold(ResidFoldArgs a, long ncp
     // Uz loader: k = tid/16 (z' = k0 + k), y = (tid%16)*4..+3;  Uy loader: z = tid/4, k = (tid%4)*4..+3 (y')
     const int uz_k = tid >> 4, uz_y = (tid & 15) * 4;
     const int uy_z = tid >> 2, uy_k = (tid & 3) * 4;
-    double rz[2][4], ry[2][4];
+    f4v vz[NSRC][2], vy[NSRC][2];           // this thread's FP32 source values of the next stage
     auto load = [&](int ks) {
         const int k0 = ks * RKT;
 #pragma unroll
-        for (int part = 0; part < 2; ++part) {
-            // sums in FP64 of the FP32 sources
-            double sz[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0};
-            for (int t = 0; t < a.nsrc; ++t) {
-                const f4v vz = ((const f4v*)(a.src[t] + qbase[part] + (size_t)(k0 + uz_k) * NH + y0 + uz_y))[0];
-                const f4v vy = ((const f4v*)(a.src[t] + qbase[part] + (size_t)(z0 + uy_z) * NH + k0 + uy_k))[0];
+        for (int t = 0; t < NSRC; ++t)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    sz[e] += (double)vz[e];
-                    sy[e] += (double)vy[e];
-                }
+            for (int part = 0; part < 2; ++part) {
+                vz[t][part] = ((const f4v*)(a.src[t] + qbase[part] + (size_t)(k0 + uz_k) * NH + y0 + uz_y))[0];
+                vy[t][part] = ((const f4v*)(a.src[t] + qbase[part] + (size_t)(z0 + uy_z) * NH + k0 + uy_k))[0];
             }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                rz[part][e] = sz[e];
-                ry[part][e] = sy[e];
-            }
-        }
     };
     auto store = [&](int buf) {
 #pragma unroll
         for (int part = 0; part < 2; ++part)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                Uz[buf][part][(uz_y + e) * RLD + uz_k] = rz[part][e];
-                Uy[buf][part][uy_z * RLD + uy_k + e] = ry[part][e];
+                double sz = 0.0, sy = 0.0;       // sums in FP64 of the FP32 sources
+#pragma unroll
+                for (int t = 0; t < NSRC; ++t) {
+                    sz += (double)vz[t][part][e];
+                    sy += (double)vy[t][part][e];
+                }
+                Uz[buf][part][(uz_y + e) * RLD + uz_k] = sz;
+                Uy[buf][part][uy_z * RLD + uy_k + e] = sy;
             }
     };
     const d2v* Sr = (const d2v*)a.Sfrag[r];
@@ -518,25 +516,45 @@ __global__ __launch_bounds__(256, 2) void k_resid_fold(ResidFoldArgs a, long ncp
     const d2v* F = g.in[f] + (size_t)mode * m * m;
     const double sr = r ? -1.0 : 1.0, sc = c ? -1.0 : 1.0;
     double rmx = 0.0, fmx = 0.0;
+    // One 16 x 16 output tile at a time: its f and u operands are all loaded before its first store
+    // (R and the sources may be the same allocation for the compiler, which would otherwise keep
+    // every load behind the previous store). Points outside the quarter read its first point.
 #pragma unroll
     for (int zt = 0; zt < 2; ++zt)
 #pragma unroll
-        for (int yt = 0; yt < 2; ++yt)
+        for (int yt = 0; yt < 2; ++yt) {
+            d2v fq[4][4];
+            float us[4][NSRC][2];
+            bool in[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int z = z0 + wz * 32 + zt * 16 + 4 * e + l4;
+                const int y = y0 + wy * 32 + yt * 16 + l16;
+                in[e] = y < hh[r] && z < hh[c];
+                const int yc = in[e] ? y : 0, zc = in[e] ? z : 0;
+                const int ya = yc + 1, yb = n - yc, za = zc + 1, zb = n - zc;
+                fq[e][0] = F[ya + (size_t)za * m];
+                fq[e][1] = F[yb + (size_t)za * m];
+                fq[e][2] = F[ya + (size_t)zb * m];
+                fq[e][3] = F[yb + (size_t)zb * m];
+#pragma unroll
+                for (int t = 0; t < NSRC; ++t)
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) us[e][t][p] = a.src[t][qbase[p] + (size_t)zc * NH + yc];
+            }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int z = z0 + wz * 32 + zt * 16 + 4 * e + l4;
                 const int y = y0 + wy * 32 + yt * 16 + l16;
                 d2v rv = {0.0, 0.0};
-                if (y < hh[r] && z < hh[c]) {
-                    const int ya = y + 1, yb = n - y, za = z + 1, zb = n - z;
-                    const d2v fa = F[ya + (size_t)za * m], fb = F[yb + (size_t)za * m];
-                    const d2v fc = F[ya + (size_t)zb * m], fd = F[yb + (size_t)zb * m];
-                    const d2v fv = 0.25 * ((fa + sr * fb) + sc * (fc + sr * fd));
+                if (in[e]) {
+                    const d2v fv = 0.25 * ((fq[e][0] + sr * fq[e][1]) + sc * (fq[e][2] + sr * fq[e][3]));
                     d2v u;
 #pragma unroll
                     for (int p = 0; p < 2; ++p) {
                         double s = 0.0;
-                        for (int t = 0; t < a.nsrc; ++t) s += (double)a.src[t][qbase[p] + (size_t)z * NH + y];
+#pragma unroll
+                        for (int t = 0; t < NSRC; ++t) s += (double)us[e][t][p];
                         u[p] = s;
                     }
                     const d2v lu = {acc[zt][yt][0][e], acc[zt][yt][1][e]};
@@ -548,6 +566,7 @@ __global__ __launch_bounds__(256, 2) void k_resid_fold(ResidFoldArgs a, long ncp
 #pragma unroll
                 for (int p = 0; p < 2; ++p) a.R[qbase[p] + (size_t)z * NH + y] = (float)rv[p];
             }
+        }
     if (a.stats) {
 #pragma unroll
         for (int sh = 32; sh > 0; sh >>= 1) {
@@ -568,7 +587,14 @@ hipError_t launch_resid_fold(const ResidFoldArgs& a, hipStream_t st)
     if (ncplx <= 0) return hipSuccess;
     const int bpd = a.NH / RB, tpp = 4 * bpd * bpd;
     const long groups = (ncplx + kXcd - 1) / kXcd;
-    hipLaunchKernelGGL(k_resid_fold, dim3((unsigned)(groups * kXcd * tpp)), dim3(256), 0, st, a, ncplx);
+    const dim3 grid((unsigned)(groups * kXcd * tpp)), block(256);
+    switch (a.nsrc) {
+        case 1: hipLaunchKernelGGL(k_resid_fold<1>, grid, block, 0, st, a, ncplx); break;
+        case 2: hipLaunchKernelGGL(k_resid_fold<2>, grid, block, 0, st, a, ncplx); break;
+        case 3: hipLaunchKernelGGL(k_resid_fold<3>, grid, block, 0, st, a, ncplx); break;
+        case 4: hipLaunchKernelGGL(k_resid_fold<4>, grid, block, 0, st, a, ncplx); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

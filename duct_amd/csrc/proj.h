@@ -91,7 +91,10 @@ struct BoundaryArgs {
     const double* DC;   // m x m column-major
     const double* AM1;  // 4 x 4 column-major
     int mode;           // BoundaryMode
+    d2v* part;          // BND_NEUMANN: scratch of boundary_scratch_bytes(g) for the row sums
 };
+constexpr int kBndSplit = 8;   // workgroups per plane of the Neumann edge sums
+size_t boundary_scratch_bytes(const FieldGroup& g);
 hipError_t launch_boundary(const BoundaryArgs& a, hipStream_t st);
 
 // Neumann k=0 gauge (solvediag.f90:161-206) on planar G after pass 2.
