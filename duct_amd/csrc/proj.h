@@ -10,7 +10,7 @@ enum ProjMode : int {
     PROJ_P2P_DIV = 2,  // planar -> planar with the eigenvalue divide (pass 2)
     PROJ_P2C = 3,      // planar -> complex field interior  (pass 4)
     PROJ_P2P_LDIV = 4, // planar -> planar, the eigenvalue divide applied to the
-                       // input as it is loaded (pass 3 after an undivided pass 2)
+                       // input as it is staged in LDS (pass 3 after an undivided pass 2)
 };
 
 // Workgroup geometry of the projection kernels for a padded interior NP.

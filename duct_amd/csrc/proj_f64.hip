@@ -11,11 +11,11 @@
 // Storing the product transposed is what makes the chain close:
 //   pass 1 (C2P)      Y1 = (P^-1 F)^T                 complex field  -> planar
 //   pass 2 (P2P)      Y2 = (P^-1 Y1)^T                planar -> planar
-//   pass 3 (P2P_LDIV) Y3 = (P (Y2 ./ den))^T          planar -> planar, divide on load
+//   pass 3 (P2P_LDIV) Y3 = (P (Y2 ./ den))^T          planar -> planar, divide on the way into LDS
 //   pass 4 (P2C)      Y4 = (P Y3)^T = P G P^T = U     planar -> complex field
 // The complex<->real split of int2d3d.f90 (com2re/re2com) is fused into the
 // loads of pass 1 and the stores of pass 4; the divide of :61-66 / :151-159 is
-// applied to pass 3's input as it is loaded (the same rounding as dividing in
+// applied to pass 3's input as it is staged in LDS (the same rounding as dividing in
 // pass 2's epilogue, where it would stall every wave at once).  P2P_DIV, the
 // divide in pass 2's epilogue, serves the adjoint and the mixed solve, whose
 // gauge terms act on the divided G.
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
     int v_dst = 0, v_dst_im = 0, kk0 = 0;
     bool src_ok = false, zero_im = false;
 
-    // LDIV: column j of plane `plane` is divided by lam_j + lam_k - xlambda on load
+    // LDIV: column j of plane `plane` is divided by lam_j + lam_k - xlambda as it is staged in LDS
     int ld_j = 0;
     double ld_xlam = 0.0, ld_lamj = 0.0;
     bool ld_gauge = false;

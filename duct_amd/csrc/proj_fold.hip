@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
     int v_dst = 0, kk0 = 0;
     bool src_ok = false;
     // LDIV: column l (parity cp) of the undivided pass-2 output, row k of parity rp,
-    // is divided by lam_rp(k) + lam_cp(l) - xlambda as it is loaded
+    // is divided by lam_rp(k) + lam_cp(l) - xlambda as it is staged in LDS
     int ld_l = 0, ld_cp = 0;
     double ld_xlam = 0.0;
     bool ld_gauge = false;

@@ -54,7 +54,35 @@ def random_field(m, M, seed, dtype=np.complex128):
     return (rng.standard_normal((M, m, m)) + 1j * rng.standard_normal((M, m, m))).astype(dtype)
 
 
-@pytest.mark.parametrize("m,M", [(5, 3), (9, 4), (33, 16), (34, 5), (65, 6), (129, 4), (257, 2), (300, 2)])
+def neumann_check_other_basis(dev, ob, f, got, ref):
+    """The singular k = 0 Poisson plane is pinned at the grid node iofzero + 1, iofzero being the
+    position of the smallest eigenvalue in the eigensolver's output order (solvediag.f90:161-206).
+    Two eigensolves of the same operator may order the eigenvalues differently (seen at m = 514 on a
+    16-thread host); they are then two different rounding paths, and the two solutions differ by a
+    constant on the k = 0 plane (its null space) plus the basis' own sensitivity (1.8e-10 for a
+    1-ulp change of the operator at m = 514). Checked here: the product's gauge node holds
+    `wanted`, the difference is a constant to 1e-9; then the solve on the oracle's own basis,
+    uploaded through duct_setup_, agrees with the oracle to the FP64 bar."""
+    m = ob.b.m
+    s = dev.solver._state["setup"]
+    io = s.neumann.iofzero
+    print("gauge nodes differ: product %d, oracle %d" % (io, ob.b.iofzero), flush=True)
+    assert abs(got[0, io, io] - s.neumann.wanted) <= TOL * np.abs(got[0]).max()
+    d = got[0] - ref[0]
+    assert np.abs(d - d[0, 0]).max() <= 1e-9 * np.abs(ref[0]).max()
+    got[0] -= d[0, 0]
+    assert relerr(got, ref) <= 1e-9
+    dev.solver.upload_basis("n", m, ob.b.PN, ob.b.PNM1, ob.b.EN, ob.b.DC, ob.b.AM1, ob.b.iofzero, ob.b.wanted)
+    try:
+        same, _ = gpu_helm3d(dev, f, "n", 0.0)
+    finally:
+        dev.solver.upload_basis("n", m, s.neumann.P, s.neumann.Pinv, s.neumann.lam, s.DC, s.neumann.AM1,
+                                s.neumann.iofzero, s.neumann.wanted)
+    assert relerr(same, ref) <= TOL, relerr(same, ref)
+
+
+@pytest.mark.parametrize("m,M", [(5, 3), (9, 4), (33, 16), (34, 5), (65, 6), (129, 4), (257, 2), (300, 2),
+                                 (514, 2)])
 @pytest.mark.parametrize("bound", ["d", "n"])
 def test_helm3d_matches_oracle(dev, oracle_mod, m, M, bound):
     ob = ensure(dev, oracle_mod, m)
@@ -62,7 +90,10 @@ def test_helm3d_matches_oracle(dev, oracle_mod, m, M, bound):
     alam = 1000.0 if bound == "d" else 0.0
     ref, ref_rhs = oracle_mod.helm3d(ob, f, bound, alam, alp=1.0, ib=0)
     got, got_rhs = gpu_helm3d(dev, f, bound, alam)
-    assert relerr(got, ref) <= TOL, (m, M, bound, relerr(got, ref))
+    if bound == "n" and dev.solver._state["setup"].neumann.iofzero != ob.b.iofzero:
+        neumann_check_other_basis(dev, ob, f, got, ref)
+    else:
+        assert relerr(got, ref) <= TOL, (m, M, bound, relerr(got, ref))
     assert np.array_equal(got_rhs, ref_rhs)        # in-place boundary zeroing of 'd'
     assert not got[0].imag.any()                   # Im not solved at k = 0
 
