@@ -39,22 +39,41 @@ __global__ __launch_bounds__(256, 2) void k_xprod(XprodArgs a)
     const int yblocks = (m + kLines - 1) / kLines;
     const int z = blockIdx.x / yblocks, y0 = (blockIdx.x % yblocks) * kLines;
     const size_t mm = (size_t)m * m;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar branches only
     const int l16 = lane & 15, l4 = lane >> 4;
 
     // LDS: the coefficient tiles [3][Q][16] (step B), then product tiles [3][X][kLdp] (step D)
     __shared__ double smem[kSmem];
 
+    // Every phase issues its loads ahead of their use: the coefficients all at once, the DFT
+    // fragments one k-step ahead. A loop that loads and consumes in the same iteration waits out
+    // the full L2 / HBM latency on every step, which made this kernel latency-bound.
     // ---- A: coefficients of u, v, w for 16 lines -> coef[f][q][line]
-    for (int f = 0; f < 3; ++f) {
-        const d2v* src = a.in[f] + (size_t)z * m + y0;
-        for (int e = tid; e < M * kLines; e += 256) {
-            const int k = e / kLines, l = e % kLines;
-            d2v c = {0.0, 0.0};
-            if (y0 + l < m) c = src[(size_t)k * mm + l];
-            smem[((size_t)f * Q + k) * kLines + l] = c[0];
-            smem[((size_t)f * Q + M + k) * kLines + l] = c[1];
+    constexpr int kAIt = kXprodMaxM * kLines / 256;   // loads per thread and field
+    {
+        d2v cv[3][kAIt];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const d2v* src = a.in[f] + (size_t)z * m + y0;
+#pragma unroll
+            for (int it = 0; it < kAIt; ++it) {
+                const int e = tid + 256 * it, k = e / kLines, l = e % kLines;
+                const bool ok = e < M * kLines && y0 + l < m;
+                cv[f][it] = src[ok ? (size_t)k * mm + l : 0];
+                if (!ok) cv[f][it] = d2v{0.0, 0.0};
+            }
         }
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+#pragma unroll
+            for (int it = 0; it < kAIt; ++it) {
+                const int e = tid + 256 * it, k = e / kLines, l = e % kLines;
+                if (e < M * kLines) {
+                    smem[((size_t)f * Q + k) * kLines + l] = cv[f][it][0];
+                    smem[((size_t)f * Q + M + k) * kLines + l] = cv[f][it][1];
+                }
+            }
     }
     __syncthreads();
 
@@ -67,19 +86,38 @@ __global__ __launch_bounds__(256, 2) void k_xprod(XprodArgs a)
 #pragma unroll
         for (int s = 0; s < kMaxXT; ++s) acc[f][s] = d4v{0.0, 0.0, 0.0, 0.0};
     const double* Ei = a.Einv;                // frag: [(xt * (Q/4) + ks) * 64 + lane]
-    for (int ks = 0; ks < Q / 4; ++ks) {
-        double av[3];
+    const int KB = Q / 4;
+    const double* Eit[kMaxXT];                // this wave's x-tiles (past XT: the last tile, unused)
 #pragma unroll
-        for (int f = 0; f < 3; ++f) av[f] = smem[((size_t)f * Q + ks * 4 + l4) * kLines + l16];
+    for (int s = 0; s < kMaxXT; ++s) {
+        const int xt = wave + 4 * s;
+        Eit[s] = Ei + (size_t)(xt < XT ? xt : XT - 1) * KB * 64 + lane;
+    }
+    double bcur[kMaxXT], bnext[kMaxXT], av[3], avn[3];
+#pragma unroll
+    for (int s = 0; s < kMaxXT; ++s) bcur[s] = Eit[s][0];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) av[f] = smem[((size_t)f * Q + l4) * kLines + l16];
+    // unrolled by two, the rotation of the prefetched operands is register renaming: a copy at the
+    // bottom of the loop would wait for the loads it rotates in
+#pragma unroll 2
+    for (int ks = 0; ks < KB; ++ks) {
+        const int kn = ks + 1 < KB ? ks + 1 : ks;
+#pragma unroll
+        for (int s = 0; s < kMaxXT; ++s) bnext[s] = Eit[s][(size_t)kn * 64];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) avn[f] = smem[((size_t)f * Q + kn * 4 + l4) * kLines + l16];
 #pragma unroll
         for (int s = 0; s < kMaxXT; ++s) {
-            const int xt = wave + 4 * s;
-            if (xt >= XT) continue;
-            const double bv = Ei[((size_t)xt * (Q / 4) + ks) * 64 + lane];
+            if (wave + 4 * s >= XT) continue;
 #pragma unroll
             for (int f = 0; f < 3; ++f)
-                acc[f][s] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[f], bv, acc[f][s], 0, 0, 0);
+                acc[f][s] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[f], bcur[s], acc[f][s], 0, 0, 0);
         }
+#pragma unroll
+        for (int s = 0; s < kMaxXT; ++s) bcur[s] = bnext[s];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) av[f] = avn[f];
     }
     __syncthreads();   // coefficient tiles are dead from here on
 
@@ -109,15 +147,27 @@ __global__ __launch_bounds__(256, 2) void k_xprod(XprodArgs a)
             d4v o[3][2];
 #pragma unroll
             for (int p = 0; p < 3; ++p) o[p][0] = o[p][1] = d4v{0.0, 0.0, 0.0, 0.0};
-            for (int ks = 0; ks < X / 4; ++ks) {
-                const double b_re = Fw[((size_t)t * (X / 4) + ks) * 64 + lane];
-                const double b_im = Fw[((size_t)(t + QT2) * (X / 4) + ks) * 64 + lane];
+            const int KD = X / 4;
+            const double* Fre = Fw + (size_t)t * KD * 64 + lane;
+            const double* Fim = Fw + (size_t)(t + QT2) * KD * 64 + lane;
+            double b_re = Fre[0], b_im = Fim[0], pv[3], pn[3];
+#pragma unroll
+            for (int p = 0; p < 3; ++p) pv[p] = smem[(size_t)p * X * kLdp + (size_t)l4 * kLdp + l16];
+#pragma unroll 2
+            for (int ks = 0; ks < KD; ++ks) {
+                const int kn = ks + 1 < KD ? ks + 1 : ks;
+                const double n_re = Fre[(size_t)kn * 64], n_im = Fim[(size_t)kn * 64];
+#pragma unroll
+                for (int p = 0; p < 3; ++p) pn[p] = smem[(size_t)p * X * kLdp + (size_t)(kn * 4 + l4) * kLdp + l16];
 #pragma unroll
                 for (int p = 0; p < 3; ++p) {
-                    const double av = smem[(size_t)p * X * kLdp + (size_t)(ks * 4 + l4) * kLdp + l16];
-                    o[p][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b_re, o[p][0], 0, 0, 0);
-                    o[p][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b_im, o[p][1], 0, 0, 0);
+                    o[p][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv[p], b_re, o[p][0], 0, 0, 0);
+                    o[p][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv[p], b_im, o[p][1], 0, 0, 0);
                 }
+                b_re = n_re;
+                b_im = n_im;
+#pragma unroll
+                for (int p = 0; p < 3; ++p) pv[p] = pn[p];
             }
             const int k = t * 16 + l16;
 #pragma unroll

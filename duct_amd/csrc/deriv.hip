@@ -98,6 +98,9 @@ __device__ __forceinline__ void cline_slab(const d2v* __restrict__ U, d2v* O, co
         for (int s = 0; s < TPW; ++s) dst[s] = Dt[s][(size_t)kp * 64];
     };
     fetch(0, df);
+    // unrolled by two, the rotation df <- dn is register renaming: a copy at the bottom of the loop
+    // would wait for the very loads it rotates in
+#pragma unroll 2
     for (int kp = 0; kp < KP; ++kp) {
         if (kp + 1 < KP) fetch(kp + 1, dn);
 #pragma unroll
