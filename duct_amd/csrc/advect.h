@@ -42,6 +42,10 @@ void build_cheb_transforms(int m, std::vector<double>& fwd, std::vector<double>&
 // The collocation first and second derivative matrices of the m CGL nodes, column-major m x m,
 // correctly rounded from an extended-precision evaluation (the derivative products' matrices).
 void build_colloc_matrices(int m, std::vector<double>& d1, std::vector<double>& d2);
+// The rows y < ceil(m/2) of D (fold1) and of D2 (fold2) with the contraction folded about the
+// centre node: [Dp | Dm] in fragment layout, each (16 ceil(ceil(m/2)/16))^2 (the derivative
+// products of k_cline_fold on the even and odd parts of their input).
+void build_colloc_folded(int m, std::vector<double>& fold1, std::vector<double>& fold2);
 void build_xdft(int M, int X, std::vector<double>& einv, std::vector<double>& fwd);
 
 }  // namespace duct

@@ -23,6 +23,7 @@
 #include <cmath>
 
 #include "advect.h"
+#include "proj.h"
 
 namespace duct {
 
@@ -337,18 +338,18 @@ void build_cheb_transforms(int m, std::vector<double>& fwd, std::vector<double>&
         }
 }
 
-void build_colloc_matrices(int m, std::vector<double>& d1, std::vector<double>& d2)
+namespace {
+// D and D2 = D D of the m CGL nodes in extended precision (column-major, D[i + k m]).
+void colloc_ld(int m, std::vector<long double>& D, std::vector<long double>& D2)
 {
     // D_ik = (c_i / c_k) (-1)^(i+k) / (x_i - x_k), x_j = -cos(pi j / N) (CHQZ (2.4.31), the matrix of
     // pseudomat.f90:90-107), evaluated in extended precision: x_i - x_k from the product
     // 2 sin(pi (i+k) / 2N) sin(pi (i-k) / 2N), which has no cancellation, the diagonal from the
-    // zero row sums, D2 = D D accumulated in extended precision; both rounded once to double.
-    // The product of the double-rounded textbook entries loses ~1e-5 relative in second derivatives
-    // at m = 129; these matrices reproduce the interpolant's derivatives to ~1e-10 of the rows' scale.
+    // zero row sums, D2 = D D accumulated in extended precision.
     const int N = m - 1;
     typedef long double X;
     const X pi = 3.141592653589793238462643383279502884L;
-    std::vector<X> D((size_t)m * m, 0.0L);
+    D.assign((size_t)m * m, 0.0L);
     for (int i = 0; i < m; ++i) {
         X rs = 0.0L;
         for (int k = 0; k < m; ++k) {
@@ -361,15 +362,64 @@ void build_colloc_matrices(int m, std::vector<double>& d1, std::vector<double>& 
         }
         D[i + (size_t)i * m] = -rs;
     }
-    d1.assign((size_t)m * m, 0.0);
-    d2.assign((size_t)m * m, 0.0);
+    D2.assign((size_t)m * m, 0.0L);
     for (int j = 0; j < m; ++j)
         for (int i = 0; i < m; ++i) {
             X acc = 0.0L;
             for (int k = 0; k < m; ++k) acc += D[i + (size_t)k * m] * D[k + (size_t)j * m];
-            d1[i + (size_t)j * m] = (double)D[i + (size_t)j * m];
-            d2[i + (size_t)j * m] = (double)acc;
+            D2[i + (size_t)j * m] = acc;
         }
+}
+}  // namespace
+
+void build_colloc_matrices(int m, std::vector<double>& d1, std::vector<double>& d2)
+{
+    // Both rounded once to double. The product of the double-rounded textbook entries loses ~1e-5
+    // relative in second derivatives at m = 129; these matrices reproduce the interpolant's
+    // derivatives to ~1e-10 of the rows' scale.
+    std::vector<long double> D, D2;
+    colloc_ld(m, D, D2);
+    d1.assign((size_t)m * m, 0.0);
+    d2.assign((size_t)m * m, 0.0);
+    for (size_t e = 0; e < (size_t)m * m; ++e) {
+        d1[e] = (double)D[e];
+        d2[e] = (double)D2[e];
+    }
+}
+
+namespace {
+void fold_rows(int m, const std::vector<long double>& A, std::vector<double>& frag)
+{
+    // Rows y < he of D (or D2) with the contraction folded about the centre:
+    //   Dp(y, k) = D(y, k) + D(y, m-1-k)  (k < ho),  Dp(y, c) = D(y, c)  (odd m, c = ho)
+    //   Dm(y, k) = D(y, k) - D(y, m-1-k)  (k < ho)
+    // each sum formed in extended precision and rounded once; zero-padded to RT 16 rows and KH
+    // columns, in the fragment layout of build_frag_rect, Dp then Dm.
+    const int he = (m + 1) / 2, ho = m / 2;
+    const int KH = (he + 15) / 16 * 16, R = KH;
+    std::vector<double> P((size_t)R * KH, 0.0), Mn((size_t)R * KH, 0.0);
+    for (int y = 0; y < he; ++y)
+        for (int k = 0; k < he; ++k) {
+            const long double a = A[y + (size_t)k * m], b = A[y + (size_t)(m - 1 - k) * m];
+            if (k < ho) {
+                P[y + (size_t)k * R] = (double)(a + b);
+                Mn[y + (size_t)k * R] = (double)(a - b);
+            } else {
+                P[y + (size_t)k * R] = (double)a;   // the centre column of odd m
+            }
+        }
+    frag.assign((size_t)2 * R * KH, 0.0);
+    build_frag_rect(P.data(), R, KH, R, frag.data());
+    build_frag_rect(Mn.data(), R, KH, R, frag.data() + (size_t)R * KH);
+}
+}  // namespace
+
+void build_colloc_folded(int m, std::vector<double>& fold1, std::vector<double>& fold2)
+{
+    std::vector<long double> D, D2;
+    colloc_ld(m, D, D2);
+    fold_rows(m, D, fold1);
+    fold_rows(m, D2, fold2);
 }
 
 void build_xdft(int M, int X, std::vector<double>& einv, std::vector<double>& fwd)

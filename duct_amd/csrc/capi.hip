@@ -114,6 +114,7 @@ struct Lib {
     double* dcfrag[2] = {nullptr, nullptr}; // DC and DC^T, FP64 fragment layout (derivative products)
     double* tfrag[2] = {nullptr, nullptr};  // Chebyshev transforms phys->cheb, cheb->phys (transyz)
     double* d2frag = nullptr;             // D2 = DC DC, FP64 fragment layout (laplacian3d)
+    double* dfold[2] = {nullptr, nullptr};  // D and D2 with the contraction folded (build_colloc_folded)
     d2v* rk = nullptr;                    // Runge-Kutta substep work fields: h1..h3, advnew1..3, phi
     size_t rk_bytes = 0;
     ShenDev shen[2];                      // Shen-Legendre solve, 'd' and 'n'
@@ -979,22 +980,22 @@ int advection_run(const d2v* const in[3], d2v* const out[3], int mm, int M, int 
     const d2v* P[6];
     for (int p = 0; p < 6; ++p) P[p] = L.prods + p * field;
     enum { UU, VV, WW, UV, UW, VW };
-    const double* D = L.dcfrag[0];
+    const double* D = L.dfold[0];   // D folded (J D J = -D)
     const double sz = 1.0 / aspect;
     {
         Timed tm(PK_DERIV);
         HIPCHK(launch_derivx(P[UU], out[0], mm, M, 0, alp, 1.0, 0.0, L.stream));
     }
-    if ((st = cline_run(mm, M, {{P[UV], out[0], D, 1.0, 1.0, 0}, {P[VV], out[1], D, 1.0, 0.0, 0},
-                                {P[WW], out[2], D, sz, 0.0, 1}})))
+    if ((st = cline_run(mm, M, {{P[UV], out[0], D, 1.0, 1.0, 0, -1}, {P[VV], out[1], D, 1.0, 0.0, 0, -1},
+                                {P[WW], out[2], D, sz, 0.0, 1, -1}})))
         return st;
     {
         Timed tm(PK_DERIV);
         HIPCHK(launch_derivx(P[UV], out[1], mm, M, 0, alp, 1.0, 1.0, L.stream));
         HIPCHK(launch_derivx(P[UW], out[2], mm, M, 0, alp, 1.0, 1.0, L.stream));
     }
-    return cline_run(mm, M, {{P[UW], out[0], D, sz, 1.0, 1}, {P[VW], out[1], D, sz, 1.0, 1},
-                             {P[VW], out[2], D, 1.0, 1.0, 0}});
+    return cline_run(mm, M, {{P[UW], out[0], D, sz, 1.0, 1, -1}, {P[VW], out[1], D, sz, 1.0, 1, -1},
+                             {P[VW], out[2], D, 1.0, 1.0, 0, -1}});
 }
 
 }  // namespace
@@ -1027,6 +1028,8 @@ int duct_shutdown_(void)
     free_ptr(L.tfrag[0]);
     free_ptr(L.tfrag[1]);
     free_ptr(L.d2frag);
+    free_ptr(L.dfold[0]);
+    free_ptr(L.dfold[1]);
     L.dc_m = L.dc_np = 0;
     free_ptr(L.rk);
     L.rk_bytes = 0;
@@ -1165,6 +1168,14 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
         free_ptr(L.d2frag);
         build_frag(d2.data(), m, m, MP, f.data());
         if ((st = up(L.d2frag, f.data(), f.size()))) return st;
+        {   // the same products in half the work (k_cline_fold)
+            std::vector<double> f1, f2;
+            build_colloc_folded(m, f1, f2);
+            free_ptr(L.dfold[0]);
+            free_ptr(L.dfold[1]);
+            if ((st = up(L.dfold[0], f1.data(), f1.size()))) return st;
+            if ((st = up(L.dfold[1], f2.data(), f2.size()))) return st;
+        }
         L.dc_m = m;
         L.dc_np = MP;
     }
@@ -1648,7 +1659,7 @@ int duct_advection_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const du
         return fail(ERR_UNSUPPORTED, "advection needs nmodes %% 16 == 0, nmodes <= %d, mgalx %% 16 == 0, "
                     "3 nmodes <= mgalx <= %d (got %d, %d)", kXprodMaxM, kXprodMaxX, M, X);
     if (!(*aspect > 0.0)) return fail(ERR_ARG, "aspect must be positive");
-    if (!L.dcfrag[0] || L.dc_m != mm)
+    if (!L.dcfrag[0] || !L.dfold[0] || L.dc_m != mm)
         return fail(ERR_NOT_SETUP, "no collocation matrix for m=%d (duct_setup_ builds it)", mm);
     const d2v* vin[3] = {(const d2v*)in[0], (const d2v*)in[1], (const d2v*)in[2]};
     d2v* vout[3] = {(d2v*)out[0], (d2v*)out[1], (d2v*)out[2]};
@@ -1784,7 +1795,7 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     if (M % 16 || M > kXprodMaxM || X % 16 || X > kXprodMaxX || X < 3 * M)
         return fail(ERR_UNSUPPORTED, "the substep's advection needs nmodes %% 16 == 0, nmodes <= %d, mgalx %% 16 == 0, "
                     "3 nmodes <= mgalx <= %d (got %d, %d)", kXprodMaxM, kXprodMaxX, M, X);
-    if (!L.dcfrag[0] || !L.d2frag || L.dc_m != mm)
+    if (!L.dfold[0] || !L.dfold[1] || L.dc_m != mm)
         return fail(ERR_NOT_SETUP, "no collocation matrices for m=%d (duct_setup_ builds them)", mm);
     for (int b = 0; b < 2; ++b)
         if (!L.basis[b].ready || L.basis[b].m != mm)
@@ -1804,8 +1815,8 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     d2v* U[3] = {(d2v*)*u, (d2v*)*v, (d2v*)*w};
     d2v* P = (d2v*)*p;
     d2v* ADV[3] = {(d2v*)*adv1, (d2v*)*adv2, (d2v*)*adv3};
-    const double* D1 = L.dcfrag[0];
-    const double* D2 = L.d2frag;
+    const double* D1 = L.dfold[0];   // D and D2 folded: J D J = -D, J D2 J = D2
+    const double* D2 = L.dfold[1];
     // (ii) adv(u^{k-1}) into advnew
     const d2v* vin[3] = {U[0], U[1], U[2]};
     if ((st = advection_run(vin, advnew, mm, M, X, *alp, asp))) return st;
@@ -1837,13 +1848,14 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     }
     // laplacian3d's y parts, then its z parts, scaled as in (vii); then grad3d's y and z parts of p
     const double sy2 = s * nu * a, sz2 = s * nu * a / (asp * asp);
-    if ((st = cline_run(mm, M, {{U[0], h[0], D2, sy2, 1.0, 0}, {U[1], h[1], D2, sy2, 1.0, 0},
-                                {U[2], h[2], D2, sy2, 1.0, 0}})))
+    if ((st = cline_run(mm, M, {{U[0], h[0], D2, sy2, 1.0, 0, 1}, {U[1], h[1], D2, sy2, 1.0, 0, 1},
+                                {U[2], h[2], D2, sy2, 1.0, 0, 1}})))
         return st;
-    if ((st = cline_run(mm, M, {{U[0], h[0], D2, sz2, 1.0, 1}, {U[1], h[1], D2, sz2, 1.0, 1},
-                                {U[2], h[2], D2, sz2, 1.0, 1}})))
+    if ((st = cline_run(mm, M, {{U[0], h[0], D2, sz2, 1.0, 1, 1}, {U[1], h[1], D2, sz2, 1.0, 1, 1},
+                                {U[2], h[2], D2, sz2, 1.0, 1, 1}})))
         return st;
-    if ((st = cline_run(mm, M, {{P, h[1], D1, -s * ab, 1.0, 0}, {P, h[2], D1, -s * ab / asp, 1.0, 1}}))) return st;
+    if ((st = cline_run(mm, M, {{P, h[1], D1, -s * ab, 1.0, 0, -1}, {P, h[2], D1, -s * ab / asp, 1.0, 1, -1}})))
+        return st;
     // (viii) u*, v*, w*: Dirichlet Helmholtz, alambda = 1/(b nu dt)
     {
         const double helco = 1.0 / (b * nu * *dt);
@@ -1858,8 +1870,8 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
         Timed t(PK_DERIV);
         HIPCHK(launch_xderiv(U[0], h[0], (long)mm * mm, M, 0, 1, *alp, sd, 0.0, L.stream));
     }
-    if ((st = cline_run(mm, M, {{U[1], h[0], D1, sd, 1.0, 0}}))) return st;
-    if ((st = cline_run(mm, M, {{U[2], h[0], D1, sd / asp, 1.0, 1}}))) return st;
+    if ((st = cline_run(mm, M, {{U[1], h[0], D1, sd, 1.0, 0, -1}}))) return st;
+    if ((st = cline_run(mm, M, {{U[2], h[0], D1, sd / asp, 1.0, 1, -1}}))) return st;
     // (x) phi: Neumann Poisson
     {
         const duct_devptr_t rhs[1] = {(duct_devptr_t)h[0]};
@@ -1873,13 +1885,13 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
         Timed t(PK_DERIV);
         HIPCHK(launch_xderiv(phi, h[0], (long)mm * mm, M, 0, 2, *alp, sl, 0.0, L.stream));
     }
-    if ((st = cline_run(mm, M, {{phi, h[0], D2, sl, 1.0, 0}}))) return st;
-    if ((st = cline_run(mm, M, {{phi, h[0], D2, sl / (asp * asp), 1.0, 1}}))) return st;
+    if ((st = cline_run(mm, M, {{phi, h[0], D2, sl, 1.0, 0, 1}}))) return st;
+    if ((st = cline_run(mm, M, {{phi, h[0], D2, sl / (asp * asp), 1.0, 1, 1}}))) return st;
     {
         Timed t(PK_DERIV);
         HIPCHK(launch_rk3_update(P, U[0], phi, h[0], (long)mm * mm, M, 0, *alp, su, L.stream));
     }
-    return cline_run(mm, M, {{phi, U[1], D1, su, 1.0, 0}, {phi, U[2], D1, su / asp, 1.0, 1}});
+    return cline_run(mm, M, {{phi, U[1], D1, su, 1.0, 0, -1}, {phi, U[2], D1, su / asp, 1.0, 1, -1}});
 }
 
 // The GPU-upload tails of pseudomaty_temp / pseudomatz_temp (pseudomat_temp.f90:2-321): the

@@ -159,6 +159,200 @@ __device__ __forceinline__ void cline_slab(const d2v* __restrict__ U, d2v* O, co
     }
 }
 
+// The same product for a matrix with J D J = s D (D: s = -1, D2: s = +1; J reverses the m CGL
+// nodes), in half the MFMA work. With u split along the contraction into its even and odd parts,
+//   ue(k) = (u(k) + u(m-1-k)) / 2,  uo(k) = (u(k) - u(m-1-k)) / 2   (k < floor(m/2)),  ue(c) = u(c),
+// the output rows y < ceil(m/2) are Pe(y) + Po(y) with Pe = Dp ue and Po = Dm uo over the folded
+// contraction (Dp, Dm of build_colloc_folded), and the mirrored rows are
+//   out(m-1-y) = s (Pe(y) - Po(y)),
+// since D ue has parity s and D uo parity -s. The slab holds ue and uo (re, im) for its 16 lines;
+// a wave owns whole tiles of output rows with both parities, so it writes y and m-1-y itself.
+template <bool DIRZ, int TPW>
+__device__ __forceinline__ void cline_slab_fold(const d2v* __restrict__ U, d2v* O, const double* Dfrag, int m, int KH,
+                                                int lb, double scal, double beta, double sgn, double* slab)
+{
+    const int RT = KH / 16, LDK = KH + 1;
+    const int he = (m + 1) / 2, ho = m / 2;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int L0 = lb * 16;
+
+    // stage ue, uo: slab[(part * 2 + parity) * 16 * LDK + line * LDK + k]
+    constexpr int IT = (16 * 16 * kWaves * TPW + kThreads - 1) / kThreads;
+    const float invK = 1.0f / (float)KH;      // e / KH below is exact: e < 2^13, KH <= 144
+    d2v va[IT], vb[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * kThreads;
+        int l, k;
+        if constexpr (DIRZ) {
+            l = e & 15;
+            k = e >> 4;
+        } else {
+            l = (int)(((float)e + 0.5f) * invK);
+            k = e - l * KH;
+        }
+        const bool line = e < 16 * KH && L0 + l < m;
+        const bool oka = line && k < he, okb = line && k < ho;
+        const int kb = m - 1 - k;
+        const size_t ia = DIRZ ? (size_t)(L0 + l) + (size_t)k * m : (size_t)k + (size_t)(L0 + l) * m;
+        const size_t ib = DIRZ ? (size_t)(L0 + l) + (size_t)kb * m : (size_t)kb + (size_t)(L0 + l) * m;
+        va[it] = U[oka ? ia : 0];
+        vb[it] = U[okb ? ib : 0];
+        if (!oka) va[it] = d2v{0.0, 0.0};
+        if (!okb) vb[it] = d2v{0.0, 0.0};
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * kThreads;
+        if (e < 16 * KH) {
+            int l, k;
+            if constexpr (DIRZ) {
+                l = e & 15;
+                k = e >> 4;
+            } else {
+                l = (int)(((float)e + 0.5f) * invK);
+                k = e - l * KH;
+            }
+            d2v ue, uo;
+            if (k < ho) {
+                ue = 0.5 * (va[it] + vb[it]);
+                uo = 0.5 * (va[it] - vb[it]);
+            } else {
+                ue = va[it];                       // the centre node of odd m, or zero padding
+                uo = d2v{0.0, 0.0};
+            }
+            slab[(0 * 16 + l) * LDK + k] = ue[0];
+            slab[(1 * 16 + l) * LDK + k] = uo[0];
+            slab[(2 * 16 + l) * LDK + k] = ue[1];
+            slab[(3 * 16 + l) * LDK + k] = uo[1];
+        }
+    }
+    __syncthreads();
+
+    const d2v* Dp = (const d2v*)Dfrag;
+    const d2v* Dm = Dp + (size_t)KH * KH / 2;
+    const int KP = KH / 8;
+    d4v acc[TPW][4];                              // Pe re, Pe im, Po re, Po im
+#pragma unroll
+    for (int s = 0; s < TPW; ++s)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[s][q] = d4v{0.0, 0.0, 0.0, 0.0};
+    const d2v* Pt[TPW];
+    const d2v* Mt[TPW];
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        const int t = wave + kWaves * s, tc = t < RT ? t : RT - 1;
+        Pt[s] = Dp + (size_t)tc * KP * 64 + lane;
+        Mt[s] = Dm + (size_t)tc * KP * 64 + lane;
+    }
+    d2v fp[TPW], fm[TPW], np[TPW], nm[TPW];
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        fp[s] = Pt[s][0];
+        fm[s] = Mt[s][0];
+    }
+#pragma unroll 2
+    for (int kp = 0; kp < KP; ++kp) {
+        const int kn = kp + 1 < KP ? kp + 1 : kp;
+#pragma unroll
+        for (int s = 0; s < TPW; ++s) {
+            np[s] = Pt[s][(size_t)kn * 64];
+            nm[s] = Mt[s][(size_t)kn * 64];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = (kp * 2 + h) * 4 + l4;
+            const double er = slab[(0 * 16 + l16) * LDK + k], orr = slab[(1 * 16 + l16) * LDK + k];
+            const double ei = slab[(2 * 16 + l16) * LDK + k], oi = slab[(3 * 16 + l16) * LDK + k];
+#pragma unroll
+            for (int s = 0; s < TPW; ++s) {
+                if constexpr (!DIRZ) {
+                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, fp[s][h], acc[s][0], 0, 0, 0);
+                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, fp[s][h], acc[s][1], 0, 0, 0);
+                    acc[s][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(orr, fm[s][h], acc[s][2], 0, 0, 0);
+                    acc[s][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(oi, fm[s][h], acc[s][3], 0, 0, 0);
+                } else {
+                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fp[s][h], er, acc[s][0], 0, 0, 0);
+                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fp[s][h], ei, acc[s][1], 0, 0, 0);
+                    acc[s][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(fm[s][h], orr, acc[s][2], 0, 0, 0);
+                    acc[s][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(fm[s][h], oi, acc[s][3], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < TPW; ++s) {
+            fp[s] = np[s];
+            fm[s] = nm[s];
+        }
+    }
+
+    // epilogue: D[i][j] in lane j + 16 (i % 4), element i / 4. Row r of the folded output (r < he)
+    // is y = r (!DIRZ) or z = r (DIRZ); its mirror m-1-r exists for r < ho. All loads first.
+    d2v o1[TPW][4], o2[TPW][4];
+    if (beta != 0.0) {
+#pragma unroll
+        for (int s = 0; s < TPW; ++s) {
+            const int t = wave + kWaves * s;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = 4 * e + l4;
+                const int r = DIRZ ? 16 * t + i : 16 * t + l16;       // folded output row
+                const int c = DIRZ ? L0 + l16 : L0 + i;               // line (the other direction)
+                const bool v1 = t < RT && r < he && c < m, v2 = t < RT && r < ho && c < m;
+                const size_t a1 = DIRZ ? (size_t)c + (size_t)r * m : (size_t)r + (size_t)c * m;
+                const size_t a2 = DIRZ ? (size_t)c + (size_t)(m - 1 - r) * m : (size_t)(m - 1 - r) + (size_t)c * m;
+                o1[s][e] = v1 ? O[a1] : d2v{0.0, 0.0};
+                o2[s][e] = v2 ? O[a2] : d2v{0.0, 0.0};
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        const int t = wave + kWaves * s;
+        if (t >= RT) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = 4 * e + l4;
+            const int r = DIRZ ? 16 * t + i : 16 * t + l16;
+            const int c = DIRZ ? L0 + l16 : L0 + i;
+            if (c >= m) continue;
+            const d2v pe = {acc[s][0][e], acc[s][1][e]}, po = {acc[s][2][e], acc[s][3][e]};
+            if (r < he) {
+                d2v v = scal * (pe + po);
+                if (beta != 0.0) v += beta * o1[s][e];
+                O[DIRZ ? (size_t)c + (size_t)r * m : (size_t)r + (size_t)c * m] = v;
+            }
+            if (r < ho) {
+                d2v v = (scal * sgn) * (pe - po);
+                if (beta != 0.0) v += beta * o2[s][e];
+                O[DIRZ ? (size_t)c + (size_t)(m - 1 - r) * m : (size_t)(m - 1 - r) + (size_t)c * m] = v;
+            }
+        }
+    }
+}
+
+template <int TPW>
+__global__ __launch_bounds__(kThreads, 2) void k_cline_fold(ClineArgs a)
+{
+    __shared__ double slab[4 * 16 * (16 * kWaves * TPW + 1)];   // [part][parity][line][k], stride KH + 1
+    const int NT = a.NP / 16, m = a.m;
+    const int KH = ((m + 1) / 2 + 15) / 16 * 16;
+    const long per_op = (long)a.nmodes * NT;
+    const int o = (int)(blockIdx.x / per_op);
+    const long r = blockIdx.x % per_op;
+    const int mode = (int)(r / NT), lb = (int)(r % NT);
+    const ClineOp& op = a.op[o];
+    const d2v* U = op.in + (size_t)mode * m * m;
+    d2v* O = op.out + (size_t)mode * m * m;
+    const double sgn = op.fold < 0 ? -1.0 : 1.0;
+    if (op.dirz)
+        cline_slab_fold<true, TPW>(U, O, op.Dfrag, m, KH, lb, op.scal, op.beta, sgn, slab);
+    else
+        cline_slab_fold<false, TPW>(U, O, op.Dfrag, m, KH, lb, op.scal, op.beta, sgn, slab);
+}
+
 template <int TPW>
 __global__ __launch_bounds__(kThreads, 2) void k_cline(ClineArgs a)
 {
@@ -187,9 +381,21 @@ hipError_t launch_cline_ops(const ClineArgs& a, hipStream_t st)
         for (int j = 0; j < a.nops; ++j)   // every output is written by one op and read by no other
             if ((j != i && a.op[j].out == a.op[i].out) || a.op[j].in == a.op[i].out) return hipErrorInvalidValue;
     }
+    for (int i = 1; i < a.nops; ++i)
+        if ((a.op[i].fold != 0) != (a.op[0].fold != 0)) return hipErrorInvalidValue;
     if (a.nops == 0 || a.nmodes <= 0) return hipSuccess;
     const int NT = a.NP / 16;
     const long blocks = (long)a.nops * a.nmodes * NT;
+    if (a.op[0].fold != 0) {   // folded rows: RT = ceil(ceil(m/2) / 16) tiles, TPW = ceil(RT / 3) <= 3
+        const int RT = ((a.m + 1) / 2 + 15) / 16;
+        switch ((RT + kWaves - 1) / kWaves) {
+            case 1: hipLaunchKernelGGL(k_cline_fold<1>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+            case 2: hipLaunchKernelGGL(k_cline_fold<2>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+            case 3: hipLaunchKernelGGL(k_cline_fold<3>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch ((NT + kWaves - 1) / kWaves) {
         case 1: hipLaunchKernelGGL(k_cline<1>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
         case 2: hipLaunchKernelGGL(k_cline<2>, dim3((unsigned)blocks), dim3(kThreads), 0, st, a); break;
@@ -210,7 +416,7 @@ hipError_t launch_cline(int dirz, const CollocArgs& a, hipStream_t st)
     c.nmodes = a.g.nmodes;
     c.m = a.g.m;
     c.NP = a.NP;
-    for (int f = 0; f < a.g.nf; ++f) c.op[f] = ClineOp{a.g.in[f], a.g.out[f], a.Dfrag, a.scal, a.beta, dirz};
+    for (int f = 0; f < a.g.nf; ++f) c.op[f] = ClineOp{a.g.in[f], a.g.out[f], a.Dfrag, a.scal, a.beta, dirz, 0};
     return launch_cline_ops(c, st);
 }
 

@@ -62,9 +62,12 @@ constexpr int kMaxClineOps = 8;
 struct ClineOp {
     const d2v* in;
     d2v* out;
-    const double* Dfrag;          // the matrix in the fragment layout of build_frag, NP padded
+    const double* Dfrag;          // fold 0: the matrix in the fragment layout of build_frag, NP padded;
+                                  // fold +-1: [Dp | Dm] of build_colloc_folded
     double scal, beta;
     int dirz;
+    int fold;                     // 0, or the parity s of a matrix with J D J = s D (J the reflection
+                                  // of the CGL nodes): -1 for D, +1 for D2; all ops of a launch alike
 };
 struct ClineArgs {
     ClineOp op[kMaxClineOps];
