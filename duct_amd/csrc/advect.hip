@@ -95,30 +95,31 @@ __global__ __launch_bounds__(256, 2) void k_xprod(XprodArgs a)
         Eit[s] = Ei + (size_t)(xt < XT ? xt : XT - 1) * KB * 64 + lane;
     }
     double bcur[kMaxXT], bnext[kMaxXT], av[3], avn[3];
-#pragma unroll
-    for (int s = 0; s < kMaxXT; ++s) bcur[s] = Eit[s][0];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) av[f] = smem[((size_t)f * Q + l4) * kLines + l16];
-    // unrolled by two, the rotation of the prefetched operands is register renaming: a copy at the
-    // bottom of the loop would wait for the loads it rotates in
-#pragma unroll 2
-    for (int ks = 0; ks < KB; ++ks) {
+    // One k-step: prefetch the next step's operands into (xb, xa), then the products with (cb, ca).
+    // Called twice per iteration with the buffers swapped (KB = M / 2 is even), so the prefetched
+    // operands are never copied: a copy at the bottom of the loop would wait for the loads it rotates
+    // in, and the compiler does not unroll loops holding MFMAs when that would need a remainder.
+    auto stepB = [&](int ks, double (&cb)[kMaxXT], double (&ca)[3], double (&xb)[kMaxXT], double (&xa)[3]) {
         const int kn = ks + 1 < KB ? ks + 1 : ks;
 #pragma unroll
-        for (int s = 0; s < kMaxXT; ++s) bnext[s] = Eit[s][(size_t)kn * 64];
+        for (int s = 0; s < kMaxXT; ++s) xb[s] = Eit[s][(size_t)kn * 64];
 #pragma unroll
-        for (int f = 0; f < 3; ++f) avn[f] = smem[((size_t)f * Q + kn * 4 + l4) * kLines + l16];
+        for (int f = 0; f < 3; ++f) xa[f] = smem[((size_t)f * Q + kn * 4 + l4) * kLines + l16];
 #pragma unroll
         for (int s = 0; s < kMaxXT; ++s) {
             if (wave + 4 * s >= XT) continue;
 #pragma unroll
             for (int f = 0; f < 3; ++f)
-                acc[f][s] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[f], bcur[s], acc[f][s], 0, 0, 0);
+                acc[f][s] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[f], cb[s], acc[f][s], 0, 0, 0);
         }
+    };
 #pragma unroll
-        for (int s = 0; s < kMaxXT; ++s) bcur[s] = bnext[s];
+    for (int s = 0; s < kMaxXT; ++s) bcur[s] = Eit[s][0];
 #pragma unroll
-        for (int f = 0; f < 3; ++f) av[f] = avn[f];
+    for (int f = 0; f < 3; ++f) av[f] = smem[((size_t)f * Q + l4) * kLines + l16];
+    for (int ks = 0; ks < KB; ks += 2) {
+        stepB(ks, bcur, av, bnext, avn);
+        stepB(ks + 1, bnext, avn, bcur, av);
     }
     __syncthreads();   // coefficient tiles are dead from here on
 
@@ -151,24 +152,26 @@ __global__ __launch_bounds__(256, 2) void k_xprod(XprodArgs a)
             const int KD = X / 4;
             const double* Fre = Fw + (size_t)t * KD * 64 + lane;
             const double* Fim = Fw + (size_t)(t + QT2) * KD * 64 + lane;
-            double b_re = Fre[0], b_im = Fim[0], pv[3], pn[3];
-#pragma unroll
-            for (int p = 0; p < 3; ++p) pv[p] = smem[(size_t)p * X * kLdp + (size_t)l4 * kLdp + l16];
-#pragma unroll 2
-            for (int ks = 0; ks < KD; ++ks) {
+            double bv[2], bn[2], pv[3], pn[3];   // (re, im) fragments of Fwd; products of the lines
+            auto stepD = [&](int ks, double (&cb)[2], double (&cp)[3], double (&xb)[2], double (&xp)[3]) {
                 const int kn = ks + 1 < KD ? ks + 1 : ks;
-                const double n_re = Fre[(size_t)kn * 64], n_im = Fim[(size_t)kn * 64];
+                xb[0] = Fre[(size_t)kn * 64];
+                xb[1] = Fim[(size_t)kn * 64];
 #pragma unroll
-                for (int p = 0; p < 3; ++p) pn[p] = smem[(size_t)p * X * kLdp + (size_t)(kn * 4 + l4) * kLdp + l16];
+                for (int p = 0; p < 3; ++p) xp[p] = smem[(size_t)p * X * kLdp + (size_t)(kn * 4 + l4) * kLdp + l16];
 #pragma unroll
                 for (int p = 0; p < 3; ++p) {
-                    o[p][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv[p], b_re, o[p][0], 0, 0, 0);
-                    o[p][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv[p], b_im, o[p][1], 0, 0, 0);
+                    o[p][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cp[p], cb[0], o[p][0], 0, 0, 0);
+                    o[p][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cp[p], cb[1], o[p][1], 0, 0, 0);
                 }
-                b_re = n_re;
-                b_im = n_im;
+            };
+            bv[0] = Fre[0];
+            bv[1] = Fim[0];
 #pragma unroll
-                for (int p = 0; p < 3; ++p) pv[p] = pn[p];
+            for (int p = 0; p < 3; ++p) pv[p] = smem[(size_t)p * X * kLdp + (size_t)l4 * kLdp + l16];
+            for (int ks = 0; ks < KD; ks += 2) {   // KD = X / 4 is even
+                stepD(ks, bv, pv, bn, pn);
+                stepD(ks + 1, bn, pn, bv, pv);
             }
             const int k = t * 16 + l16;
 #pragma unroll

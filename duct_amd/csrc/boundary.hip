@@ -1,9 +1,9 @@
 // Small per-plane kernels around the projection passes:
 //   k_gauge      Neumann k=0 gauge fix          solvediag.f90:161-206
-//   k_boundary   Dirichlet zero / keep, Neumann edges + corners
-//                                               helm3d.f90:48-60, solvediag.f90:251-322
+//   k_boundary   Dirichlet zero / keep          helm3d.f90:48-60
+//   k_bnd_cols, k_bnd_rows  Neumann edges + corners  solvediag.f90:251-322
 //   k_bc_correct Dirichlet RHS correction       solvediag.f90:37-45
-// All three are bandwidth- or latency-bound and touch O(n) or O(n^2) values
+// All are bandwidth- or latency-bound and touch O(n) or O(n^2) values
 // per plane; re and im of a complex plane are processed together as double2.
 #include "common.h"
 #include "proj.h"
@@ -302,25 +302,22 @@ __global__ __launch_bounds__(kAuxThreads) void k_boundary(BoundaryArgs a)
     const size_t mm = (size_t)m * m;
     d2v* out = g.out[f] + (size_t)mode * mm;
     d2v* in = const_cast<d2v*>(g.in[f]) + (size_t)mode * mm;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-
-    if (a.mode == BND_DIR_ZERO || a.mode == BND_DIR_KEEP || a.mode == BND_OUT_ZERO) {
-        const bool zero = a.mode != BND_DIR_KEEP;
-        if (a.mode == BND_OUT_ZERO) in = out;   // leave the input field alone
-        for (int e = tid; e < 4 * m; e += kAuxThreads) {
-            const int side = e / m, t = e % m;
-            size_t idx;
-            if (side == 0) idx = (size_t)t;                            // z = 1 column
-            else if (side == 1) idx = (size_t)t + (size_t)(m - 1) * m; // z = m
-            else if (side == 2) idx = (size_t)t * m;                   // y = 1 row
-            else idx = (size_t)(m - 1) + (size_t)t * m;                // y = m row
-            d2v v = {0.0, 0.0};
-            if (!zero) v = in[idx];
-            if (zero) in[idx] = v;
-            if (g.mode0 + mode == 0) v[1] = 0.0;   // int2d3d.f90:43: Im not solved at k=0
-            out[idx] = v;
-        }
-        return;
+    const int tid = threadIdx.x;
+    // BND_DIR_ZERO, BND_DIR_KEEP, BND_OUT_ZERO (BND_NEUMANN runs k_bnd_cols + k_bnd_rows)
+    const bool zero = a.mode != BND_DIR_KEEP;
+    if (a.mode == BND_OUT_ZERO) in = out;   // leave the input field alone
+    for (int e = tid; e < 4 * m; e += kAuxThreads) {
+        const int side = e / m, t = e % m;
+        size_t idx;
+        if (side == 0) idx = (size_t)t;                            // z = 1 column
+        else if (side == 1) idx = (size_t)t + (size_t)(m - 1) * m; // z = m
+        else if (side == 2) idx = (size_t)t * m;                   // y = 1 row
+        else idx = (size_t)(m - 1) + (size_t)t * m;                // y = m row
+        d2v v = {0.0, 0.0};
+        if (!zero) v = in[idx];
+        if (zero) in[idx] = v;
+        if (g.mode0 + mode == 0) v[1] = 0.0;   // int2d3d.f90:43: Im not solved at k=0
+        out[idx] = v;
     }
 }
 

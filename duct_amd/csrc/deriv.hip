@@ -97,12 +97,13 @@ __device__ __forceinline__ void cline_slab(const d2v* __restrict__ U, d2v* O, co
 #pragma unroll
         for (int s = 0; s < TPW; ++s) dst[s] = Dt[s][(size_t)kp * 64];
     };
-    fetch(0, df);
-    // unrolled by two, the rotation df <- dn is register renaming: a copy at the bottom of the loop
-    // would wait for the very loads it rotates in
-#pragma unroll 2
-    for (int kp = 0; kp < KP; ++kp) {
-        if (kp + 1 < KP) fetch(kp + 1, dn);
+    // One fragment pair: prefetch the next pair's fragments into `nxt`, then the products with `cur`.
+    // The loop below calls it twice per iteration with the buffers swapped (KP = NP / 8 is even), so
+    // the prefetched fragments are never copied: a copy at the bottom of the loop would wait for the
+    // very loads it rotates in (and the compiler does not unroll loops holding MFMAs when that would
+    // need a remainder).
+    auto step = [&](int kp, d2v (&cur)[TPW], d2v (&nxt)[TPW]) {
+        fetch(kp + 1 < KP ? kp + 1 : kp, nxt);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int k = (kp * 2 + h) * 4 + l4;
@@ -111,17 +112,20 @@ __device__ __forceinline__ void cline_slab(const d2v* __restrict__ U, d2v* O, co
             for (int s = 0; s < TPW; ++s) {
                 if constexpr (!DIRZ) {
                     // acc[i = z][j = y] = sum_k u(k, z) D(y, k): A = slab [z][k], B = D fragment [k][y]
-                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ure, df[s][h], acc[s][0], 0, 0, 0);
-                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(uim, df[s][h], acc[s][1], 0, 0, 0);
+                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ure, cur[s][h], acc[s][0], 0, 0, 0);
+                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(uim, cur[s][h], acc[s][1], 0, 0, 0);
                 } else {
                     // acc[i = z][j = y] = sum_k D(z, k) u(y, k): A = D fragment [z][k], B = slab [k][y]
-                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(df[s][h], ure, acc[s][0], 0, 0, 0);
-                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(df[s][h], uim, acc[s][1], 0, 0, 0);
+                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[s][h], ure, acc[s][0], 0, 0, 0);
+                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[s][h], uim, acc[s][1], 0, 0, 0);
                 }
             }
         }
-#pragma unroll
-        for (int s = 0; s < TPW; ++s) df[s] = dn[s];
+    };
+    fetch(0, df);
+    for (int kp = 0; kp < KP; kp += 2) {
+        step(kp, df, dn);
+        step(kp + 1, dn, df);
     }
 
     // epilogue: D[i][j] in lane j + 16 (i % 4), element i / 4; y runs along lanes (coalesced).
@@ -248,18 +252,14 @@ __device__ __forceinline__ void cline_slab_fold(const d2v* __restrict__ U, d2v* 
         Mt[s] = Dm + (size_t)tc * KP * 64 + lane;
     }
     d2v fp[TPW], fm[TPW], np[TPW], nm[TPW];
-#pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        fp[s] = Pt[s][0];
-        fm[s] = Mt[s][0];
-    }
-#pragma unroll 2
-    for (int kp = 0; kp < KP; ++kp) {
+    // as in cline_slab: one fragment pair per call, called twice per iteration with the buffers
+    // swapped (KP = KH / 8 is even)
+    auto step = [&](int kp, d2v (&cp)[TPW], d2v (&cm)[TPW], d2v (&xp)[TPW], d2v (&xm)[TPW]) {
         const int kn = kp + 1 < KP ? kp + 1 : kp;
 #pragma unroll
         for (int s = 0; s < TPW; ++s) {
-            np[s] = Pt[s][(size_t)kn * 64];
-            nm[s] = Mt[s][(size_t)kn * 64];
+            xp[s] = Pt[s][(size_t)kn * 64];
+            xm[s] = Mt[s][(size_t)kn * 64];
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -269,23 +269,27 @@ __device__ __forceinline__ void cline_slab_fold(const d2v* __restrict__ U, d2v* 
 #pragma unroll
             for (int s = 0; s < TPW; ++s) {
                 if constexpr (!DIRZ) {
-                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, fp[s][h], acc[s][0], 0, 0, 0);
-                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, fp[s][h], acc[s][1], 0, 0, 0);
-                    acc[s][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(orr, fm[s][h], acc[s][2], 0, 0, 0);
-                    acc[s][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(oi, fm[s][h], acc[s][3], 0, 0, 0);
+                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, cp[s][h], acc[s][0], 0, 0, 0);
+                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, cp[s][h], acc[s][1], 0, 0, 0);
+                    acc[s][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(orr, cm[s][h], acc[s][2], 0, 0, 0);
+                    acc[s][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(oi, cm[s][h], acc[s][3], 0, 0, 0);
                 } else {
-                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fp[s][h], er, acc[s][0], 0, 0, 0);
-                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fp[s][h], ei, acc[s][1], 0, 0, 0);
-                    acc[s][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(fm[s][h], orr, acc[s][2], 0, 0, 0);
-                    acc[s][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(fm[s][h], oi, acc[s][3], 0, 0, 0);
+                    acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cp[s][h], er, acc[s][0], 0, 0, 0);
+                    acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cp[s][h], ei, acc[s][1], 0, 0, 0);
+                    acc[s][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(cm[s][h], orr, acc[s][2], 0, 0, 0);
+                    acc[s][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(cm[s][h], oi, acc[s][3], 0, 0, 0);
                 }
             }
         }
+    };
 #pragma unroll
-        for (int s = 0; s < TPW; ++s) {
-            fp[s] = np[s];
-            fm[s] = nm[s];
-        }
+    for (int s = 0; s < TPW; ++s) {
+        fp[s] = Pt[s][0];
+        fm[s] = Mt[s][0];
+    }
+    for (int kp = 0; kp < KP; kp += 2) {
+        step(kp, fp, fm, np, nm);
+        step(kp + 1, np, nm, fp, fm);
     }
 
     // epilogue: D[i][j] in lane j + 16 (i % 4), element i / 4. Row r of the folded output (r < he)
