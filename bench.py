@@ -92,6 +92,28 @@ def copy_probe(mib, steps):
     print(json.dumps({"copy_probe_bytes_each_way": n * 8, "copies": steps}), flush=True)
 
 
+def _timed_twice(fn, steps, warmup):
+    """Warmup, then `steps` calls with nothing recorded between the launches (the time), then the
+    same `steps` again with the library's hipEvents around every launch (the kernel classes)."""
+    import torch
+    import duct_amd
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    with duct_amd.KernelTimer() as kt:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        dt_ev = (time.perf_counter() - t0) / steps
+    return dt, dt_ev, kt
+
+
 def c3_bench(steps, warmup):
     """Config 3 (129^2 CGL, 64 modes, mgalx = 192): time of advection3d and of one projection
     substep on device; the kernel classes come from the library's own hipEvents."""
@@ -113,16 +135,8 @@ def c3_bench(steps, warmup):
                      ("step", lambda: step.c3_step(u, v, w, nu=0.01, dt=0.1, mgalx=192, alp=cfg.alp)),
                      ("rk3_step", lambda: step.rk3_step(ru, rv, rw, rp, radv, fnu=0.01, dt=0.01, mgalx=192,
                                                         alp=cfg.alp))):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        with duct_amd.KernelTimer() as kt:
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                fn()
-            torch.cuda.synchronize()
-            dt = (time.perf_counter() - t0) / steps
-        out[name] = {"ms": dt * 1e3,
+        dt, dt_ev, kt = _timed_twice(fn, steps, warmup)
+        out[name] = {"ms": dt * 1e3, "ms_with_events": dt_ev * 1e3,
                      "kernels": {k: {"launches": kt.count[k] / steps, "avg_us": kt.ms[k] / kt.count[k] * 1e3}
                                  for k in kt.ms if kt.count[k]}}
     # the widened solves at the same 129^2 x 64-mode size: mixed-boundary (temperature) and Shen-Legendre
@@ -135,16 +149,8 @@ def c3_bench(steps, warmup):
     shen.setup_shen(mm, "d")
     for name, fn in (("helm3d_temp dndn", lambda: temperature.helm3d_temp(rhs, sol, bc, bc, 1000.0)),
                      ("helm3d_shen d", lambda: shen.helm3d_shen(rhs, sol, "d", 1000.0))):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        with duct_amd.KernelTimer() as kt:
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                fn()
-            torch.cuda.synchronize()
-            dt = (time.perf_counter() - t0) / steps
-        out[name] = {"ms": dt * 1e3, "mode_solves_per_s": MM / dt,
+        dt, dt_ev, kt = _timed_twice(fn, steps, warmup)
+        out[name] = {"ms": dt * 1e3, "ms_with_events": dt_ev * 1e3, "mode_solves_per_s": MM / dt,
                      "kernels": {k: {"launches": kt.count[k] / steps, "avg_us": kt.ms[k] / kt.count[k] * 1e3}
                                  for k in kt.ms if kt.count[k]}}
     lines, X, M = cfg.m * cfg.m, 192, cfg.nmodes
