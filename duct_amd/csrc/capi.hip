@@ -1808,10 +1808,12 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     const int k = *krk - 1;
     const double a = akkc[k], b = bkkc[k], nu = *fnu, ab = a + b, asp = *aspect;
     const size_t field = (size_t)M * mm * mm;
-    if ((st = ensure_buf(L.rk, L.rk_bytes, 7 * field * sizeof(d2v)))) return st;
+    if ((st = ensure_buf(L.rk, L.rk_bytes, 9 * field * sizeof(d2v)))) return st;
     d2v* h[3] = {L.rk, L.rk + field, L.rk + 2 * field};
     d2v* advnew[3] = {L.rk + 3 * field, L.rk + 4 * field, L.rk + 5 * field};
     d2v* phi = L.rk + 6 * field;
+    d2v* ty = L.rk + 7 * field;   // the y and z terms of a sum, computed in one launch before the x term
+    d2v* tz = L.rk + 8 * field;
     d2v* U[3] = {(d2v*)*u, (d2v*)*v, (d2v*)*w};
     d2v* P = (d2v*)*p;
     d2v* ADV[3] = {(d2v*)*adv1, (d2v*)*adv2, (d2v*)*adv3};
@@ -1866,12 +1868,13 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     }
     // (ix) div3d(u*, v*, w*) / ((a+b) dt) -> h1
     const double sd = 1.0 / (ab * *dt);
+    // (x + y) + z: the y and z products into ty, tz in one launch, then the x term adds them in
+    // that order (the same rounding as accumulating them into h1 one after the other)
+    if ((st = cline_run(mm, M, {{U[1], ty, D1, sd, 0.0, 0, -1}, {U[2], tz, D1, sd / asp, 0.0, 1, -1}}))) return st;
     {
         Timed t(PK_DERIV);
-        HIPCHK(launch_xderiv(U[0], h[0], (long)mm * mm, M, 0, 1, *alp, sd, 0.0, L.stream));
+        HIPCHK(launch_xderiv(U[0], h[0], (long)mm * mm, M, 0, 1, *alp, sd, 0.0, ty, tz, L.stream));
     }
-    if ((st = cline_run(mm, M, {{U[1], h[0], D1, sd, 1.0, 0, -1}}))) return st;
-    if ((st = cline_run(mm, M, {{U[2], h[0], D1, sd / asp, 1.0, 1, -1}}))) return st;
     // (x) phi: Neumann Poisson
     {
         const duct_devptr_t rhs[1] = {(duct_devptr_t)h[0]};
@@ -1881,12 +1884,12 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     }
     // (xi) update_uvwp: rhsphi = (a+b) dt nu lap(phi) -> h1; p += phi - rhsphi; u -= dt (a+b) grad phi
     const double sl = ab * *dt * nu, su = -*dt * ab;
+    if ((st = cline_run(mm, M, {{phi, ty, D2, sl, 0.0, 0, 1}, {phi, tz, D2, sl / (asp * asp), 0.0, 1, 1}})))
+        return st;
     {
         Timed t(PK_DERIV);
-        HIPCHK(launch_xderiv(phi, h[0], (long)mm * mm, M, 0, 2, *alp, sl, 0.0, L.stream));
+        HIPCHK(launch_xderiv(phi, h[0], (long)mm * mm, M, 0, 2, *alp, sl, 0.0, ty, tz, L.stream));
     }
-    if ((st = cline_run(mm, M, {{phi, h[0], D2, sl, 1.0, 0, 1}}))) return st;
-    if ((st = cline_run(mm, M, {{phi, h[0], D2, sl / (asp * asp), 1.0, 1, 1}}))) return st;
     {
         Timed t(PK_DERIV);
         HIPCHK(launch_rk3_update(P, U[0], phi, h[0], (long)mm * mm, M, 0, *alp, su, L.stream));

@@ -23,9 +23,10 @@ struct Rk3RhsArgs {
 };
 hipError_t launch_rk3_rhs(const Rk3RhsArgs& a, hipStream_t st);
 
-// out = beta out + scal (i alp k)^ider in, ider = 1 or 2 (derivx, fourier.F:117-188)
+// out = ((beta out + scal (i alp k)^ider in) + add0) + add1, ider = 1 or 2 (derivx,
+// fourier.F:117-188); add0 / add1 may be null (the y and z terms of the same sum, computed beforehand)
 hipError_t launch_xderiv(const d2v* in, d2v* out, long mm, int nmodes, int mode0, int ider, double alp, double scal,
-                         double beta, hipStream_t st);
+                         double beta, const d2v* add0, const d2v* add1, hipStream_t st);
 
 // update_uvwp's elementwise part (operators.F:564-577): p += phi - rhsphi, u += su (i alp k) phi
 hipError_t launch_rk3_update(d2v* p, d2v* u, const d2v* phi, const d2v* rhsphi, long mm, int nmodes, int mode0,

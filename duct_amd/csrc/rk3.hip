@@ -49,7 +49,8 @@ hipError_t launch_rk3_rhs(const Rk3RhsArgs& a, hipStream_t st)
 }
 
 __global__ __launch_bounds__(kThreads) void k_xderiv(const d2v* __restrict__ in, d2v* out, long mm, int nmodes,
-                                                     int mode0, int ider, double alp, double scal, double beta)
+                                                     int mode0, int ider, double alp, double scal, double beta,
+                                                     const d2v* __restrict__ add0, const d2v* __restrict__ add1)
 {
     const long total = mm * nmodes;
     for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < total; e += (long)gridDim.x * kThreads) {
@@ -57,18 +58,20 @@ __global__ __launch_bounds__(kThreads) void k_xderiv(const d2v* __restrict__ in,
         const d2v u = in[e];
         d2v v = ider == 1 ? scal * xa * d2v{-u[1], u[0]} : (-scal * xa * xa) * u;
         if (beta != 0.0) v += beta * out[e];
+        if (add0) v += add0[e];
+        if (add1) v += add1[e];
         out[e] = v;
     }
 }
 
 hipError_t launch_xderiv(const d2v* in, d2v* out, long mm, int nmodes, int mode0, int ider, double alp, double scal,
-                         double beta, hipStream_t st)
+                         double beta, const d2v* add0, const d2v* add1, hipStream_t st)
 {
     const long total = mm * nmodes;
     if (total <= 0) return hipSuccess;
     if (ider != 1 && ider != 2) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_xderiv, dim3(blocks_for(total)), dim3(kThreads), 0, st, in, out, mm, nmodes, mode0, ider,
-                       alp, scal, beta);
+                       alp, scal, beta, add0, add1);
     return hipGetLastError();
 }
 
