@@ -62,7 +62,10 @@ def run_adv(dev, fields, mgalx, alp, aspect=1.0):
     return [a.cpu().numpy() for a in adv]
 
 
-@pytest.mark.parametrize("m,M,mgalx,alp", [(17, 16, 48, 1.3), (33, 32, 96, 1.0), (40, 16, 64, 0.7), (129, 64, 192, 1.0)])
+# m = 257 and 288 run the folded derivative products with three row tiles per wave (the largest
+# instantiation; 288 is the largest m of the derivative kernels)
+@pytest.mark.parametrize("m,M,mgalx,alp", [(17, 16, 48, 1.3), (33, 32, 96, 1.0), (40, 16, 64, 0.7), (129, 64, 192, 1.0),
+                                          (257, 16, 48, 1.0), (288, 16, 48, 0.8)])
 def test_advection_matches_oracle(dev, m, M, mgalx, alp):
     from oracle import spectral
     ensure(dev, m)
