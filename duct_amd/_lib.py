@@ -48,6 +48,7 @@ SIGNATURES = {
     "duct_cyzx2xyz_dev_": (_i, [_pi, _pi, _pi, _psz, _psz]),
     "duct_mat_invdiag_dev_": (_i, [_pi, _pi, _pd, _psz, _psz, _psz]),
     "duct_set_fold_": (_i, [_pi]),
+    "duct_set_overlap_": (_i, [_pi]),
     "duct_fold_size_": (_i, [_pc, _pi, _sz]),
     "duct_prof_enable_": (_i, [_pi]),
     "duct_prof_read_": (_i, [_pi, _pd, _pi]),

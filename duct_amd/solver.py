@@ -153,6 +153,12 @@ def set_fold(mode) -> None:
     _lib.call("duct_set_fold_", _lib.byref_int(int(mode)))
 
 
+def set_overlap(on: bool = True) -> None:
+    """Run the Neumann group of a batched solve on a second stream, concurrently with the Dirichlet
+    group (duct_set_overlap_; on by default, not while kernel timing is enabled)."""
+    _lib.call("duct_set_overlap_", _lib.byref_int(1 if on else 0))
+
+
 def fold_size(bound: str) -> int:
     """Folded size NH of the basis set up for bound (0: the basis does not fold)."""
     v = ctypes.c_int(0)

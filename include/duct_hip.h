@@ -302,6 +302,16 @@ int duct_mat_invdiag_dev_(const int *Nx, const int *Ny, const double *xlam, cons
  * The reference has no such switch.                                         */
 int duct_set_fold_(const int *mode);
 
+/* Concurrency of the boundary groups of a batched solve.  on != 0 (default):
+ * when a call has both Dirichlet and Neumann fields, the Neumann group runs on
+ * a second stream owned by the library (default priority), forked from and
+ * joined back into the caller's stream with events, so its latency-bound
+ * kernels overlap the Dirichlet passes; the results are the same to the bit.
+ * Kernel timing (duct_prof_enable_) keeps the groups in order.  on = 0: both
+ * groups on the caller's stream, one after the other.  The reference solves
+ * one field per helm3d call and has no such switch.                         */
+int duct_set_overlap_(const int *on);
+
 /* Folded size NH (padded ceil(n/2)) of the basis set up for bound, 0 when it
  * does not fold (n < 65, or eigenvectors without a clean parity split).     */
 int duct_fold_size_(const char *bound, int *NH, size_t bound_len);

@@ -29,6 +29,7 @@ program use_duct
   call duct_check(duct_int2d3d(sol, rhs, 'd', 2.0_c_double, 1_c_int, 1_c_size_t), 'int2d3d')
   call duct_check(duct_solvediag(U, 'n', 2.0_c_double, 1_c_size_t), 'solvediag')
   call duct_check(duct_set_fold(1_c_int), 'fold')
+  call duct_check(duct_set_overlap(1_c_int), 'overlap')
   call duct_check(duct_helm3d_mixed_batch_dev(1_c_int, r, s, 'd', alams, m, nm, 0_c_int, 1.0_c_double, &
        3_c_int, 3_c_int, 1_c_int, c_null_ptr, 1_c_size_t), 'mixed')
   call duct_check(duct_helm3d_adjoint_batch_dev(1_c_int, r, s, 'd', alams, m, nm, 0_c_int, 1.0_c_double, &
