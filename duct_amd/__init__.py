@@ -8,6 +8,7 @@ Layout
   basis.py     one-time FP64 host eigensolve (the pseudomat step)
   solver.py    helm3d / int2d3d / solvediag mirrors over the library
   workloads.py seeded synthetic right-hand sides of the benchmark configs
+  h5io.py      reading the reference's uvwp_NNNN.h5 field files (host code in the library)
 """
 from . import _lib  # noqa: F401
 from .solver import (  # noqa: F401

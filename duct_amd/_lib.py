@@ -51,6 +51,10 @@ SIGNATURES = {
     "duct_set_overlap_": (_i, [_pi]),
     "duct_fold_size_": (_i, [_pc, _pi, _sz]),
     "duct_prof_enable_": (_i, [_pi]),
+    "duct_h5_read_header_": (_i, [_pc, _pi, _pd, _sz]),
+    "duct_h5_read_field_": (_i, [_pc, _pc, _vp, _pi, _pi, _pi, _pi, _sz, _sz]),
+    "duct_h5_read_doubles_": (_i, [_pc, _pc, _vp, ctypes.POINTER(ctypes.c_longlong), _sz, _sz]),
+    "duct_h5_read_chars_": (_i, [_pc, _pc, _vp, _pi, _pi, _sz, _sz]),
     "duct_prof_read_": (_i, [_pi, _pd, _pi]),
 }
 

@@ -5,11 +5,13 @@
 // scratch buffers of nplanes * NP^2 doubles that the projection passes
 // ping-pong through.  Host-array entry points (duct_helm3d_, duct_int2d3d_,
 // duct_solvediag_) additionally keep device copies of their fields.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
@@ -18,6 +20,7 @@
 #include "advect.h"
 #include "aux.h"
 #include "common.h"
+#include "h5read.h"
 #include "mixed_fold.h"
 #include "proj.h"
 #include "proj_lp.h"
@@ -2202,6 +2205,122 @@ int duct_helm3d_shen_dev_(const duct_devptr_t* rhs, const duct_devptr_t* sol, co
     // u = B_y V B_z^T: back to Shen, Legendre, Chebyshev and the nodes
     if ((st = prod(U, L.tfield, 2, false))) return st;
     if ((st = prod(L.tfield, U, 3, true))) return st;
+    return OK;
+}
+
+// ---- Fourier-space field files (HDF5) ------------------------------------------------------
+// Host code only: these read files into host arrays and need no device.
+
+int duct_h5_read_header_(const char* filename, int* ints, double* reals, size_t filename_len)
+{
+    if (!filename || !ints || !reals) return fail(ERR_ARG, "null argument to duct_h5_read_header_");
+    try {
+        duct_h5::File f(duct_h5::fortran_string(filename, filename_len));
+        auto scalar = [&](const char* name, bool required, double& v) {
+            if (!f.has(name)) {
+                if (required) throw std::runtime_error(std::string("no header dataset '") + name + "'");
+                v = std::nan("");
+                return;
+            }
+            const duct_h5::Dataset d = f.dataset(name);
+            if (d.count() < 1) throw std::runtime_error(std::string("empty header dataset '") + name + "'");
+            f.read_doubles(d, 0, 1, &v);
+        };
+        double v[5];
+        scalar("mgalx", true, v[0]);
+        scalar("my", true, v[2]);
+        scalar("mz", true, v[3]);
+        scalar("nparams", false, v[4]);
+        if (f.has("mx1")) scalar("mx1", true, v[1]);
+        else v[1] = 2 * ((int)v[0] / 3) / 2 - 1;        // readheader_hdf5's fallback (:824-826)
+        for (int i = 0; i < 5; ++i) ints[i] = std::isnan(v[i]) ? -1 : (int)v[i];
+        const char* names[7] = {"alp", "aspect", "time", "fnu", "cfl", "dtfixed", "cmassflow"};
+        for (int i = 0; i < 7; ++i) scalar(names[i], i == 0 || i == 2, reals[i]);
+    } catch (const std::exception& e) {
+        return fail(ERR_ARG, "%s", e.what());
+    }
+    return OK;
+}
+
+int duct_h5_read_field_(const char* filename, const char* name, double* out, const int* my, const int* mz,
+                        const int* ib, const int* nmodes, size_t filename_len, size_t name_len)
+{
+    if (!filename || !name || !out || !my || !mz || !ib || !nmodes)
+        return fail(ERR_ARG, "null argument to duct_h5_read_field_");
+    if (*my < 2 || *mz < 2 || *ib < 0 || *nmodes < 1)
+        return fail(ERR_ARG, "my=%d, mz=%d, ib=%d, nmodes=%d: need my, mz >= 2, ib >= 0, nmodes >= 1", *my, *mz,
+                    *ib, *nmodes);
+    const std::string fname = duct_h5::fortran_string(filename, filename_len);
+    const std::string dname = duct_h5::fortran_string(name, name_len);
+    try {
+        duct_h5::File f(fname);
+        const duct_h5::Dataset d = f.dataset(dname);
+        if (d.dims.size() != 4 || d.dims[3] != 2 || d.kind != duct_h5::Kind::Float)
+            return fail(ERR_ARG, "%s: '%s' is not a Fourier-space field (floating point, extents (2, my, mz, "
+                        "modes) in Fortran order)", fname.c_str(), dname.c_str());
+        const long myr = (long)d.dims[2], mzr = (long)d.dims[1], mfile = (long)d.dims[0];
+        const bool same = myr == *my && mzr == *mz;
+        if (!same && (myr == *my || mzr == *mz))
+            return fail(ERR_UNSUPPORTED, "%s: the file's grid is %ld x %ld and the caller's %d x %d; only one of "
+                        "my, mz differs, which the reference does not interpolate", fname.c_str(), myr, mzr,
+                        *my, *mz);
+        if (!same && (myr < 2 || mzr < 2))
+            return fail(ERR_ARG, "%s: grid %ld x %ld too small to interpolate", fname.c_str(), myr, mzr);
+        const long nread = std::max(0L, std::min<long>(*nmodes, mfile - *ib));   // modes past mx1 stay 0
+        const size_t plane_out = (size_t)2 * *my * *mz, plane_in = (size_t)2 * myr * mzr;
+        if (same) {
+            if (nread) f.read_doubles(d, (uint64_t)*ib * plane_in, (uint64_t)nread * plane_in, out);
+        } else {
+            std::vector<double> tmp(plane_in);
+            for (long k = 0; k < nread; ++k) {
+                f.read_doubles(d, (uint64_t)(*ib + k) * plane_in, plane_in, tmp.data());
+                duct_h5::cheb_interp((int)myr, (int)mzr, tmp.data(), *my, *mz, out + (size_t)k * plane_out);
+            }
+        }
+        std::fill(out + (size_t)nread * plane_out, out + (size_t)*nmodes * plane_out, 0.0);
+    } catch (const std::exception& e) {
+        return fail(ERR_ARG, "%s", e.what());
+    }
+    return OK;
+}
+
+int duct_h5_read_doubles_(const char* filename, const char* name, double* out, const long long* count,
+                          size_t filename_len, size_t name_len)
+{
+    if (!filename || !name || !out || !count) return fail(ERR_ARG, "null argument to duct_h5_read_doubles_");
+    const std::string fname = duct_h5::fortran_string(filename, filename_len);
+    const std::string dname = duct_h5::fortran_string(name, name_len);
+    try {
+        duct_h5::File f(fname);
+        const duct_h5::Dataset d = f.dataset(dname);
+        if ((long long)d.count() != *count)
+            return fail(ERR_ARG, "%s: '%s' holds %llu elements, not %lld", fname.c_str(), dname.c_str(),
+                        (unsigned long long)d.count(), *count);
+        f.read_doubles(d, 0, d.count(), out);
+    } catch (const std::exception& e) {
+        return fail(ERR_ARG, "%s", e.what());
+    }
+    return OK;
+}
+
+int duct_h5_read_chars_(const char* filename, const char* name, char* out, const int* capacity, int* length,
+                        size_t filename_len, size_t name_len)
+{
+    if (!filename || !name || !out || !capacity || !length)
+        return fail(ERR_ARG, "null argument to duct_h5_read_chars_");
+    const std::string fname = duct_h5::fortran_string(filename, filename_len);
+    const std::string dname = duct_h5::fortran_string(name, name_len);
+    try {
+        duct_h5::File f(fname);
+        const std::string s = f.read_string(f.dataset(dname));
+        if ((long)s.size() > *capacity)
+            return fail(ERR_ARG, "%s: '%s' holds %zu characters, more than %d", fname.c_str(), dname.c_str(),
+                        s.size(), *capacity);
+        std::memcpy(out, s.data(), s.size());
+        *length = (int)s.size();
+    } catch (const std::exception& e) {
+        return fail(ERR_ARG, "%s", e.what());
+    }
     return OK;
 }
 

@@ -14,7 +14,8 @@ module duct_hip
             duct_helm3d_mixed_batch_dev, duct_helm3d_adjoint_batch_dev, duct_colloc_deriv_dev, &
             duct_derivx_dev, duct_advection_dev, duct_transyz_dev, duct_derivyz_dev, duct_fourx_dev, &
             duct_rk3_substep_dev, duct_setup_temp, duct_helm3d_temp_dev, &
-            duct_setup_shen, duct_helm3d_shen_dev
+            duct_setup_shen, duct_helm3d_shen_dev, duct_h5_read_header, duct_h5_read_field, &
+            duct_h5_read_doubles, duct_h5_read_chars
 
   interface
     integer(c_int) function duct_init(device) bind(C, name="duct_init_")
@@ -108,6 +109,51 @@ module duct_hip
     integer(c_int) function duct_set_overlap(on) bind(C, name="duct_set_overlap_")
       import :: c_int
       integer(c_int), intent(in) :: on
+    end function
+
+    ! Field files (host code, no device needed). Strings are passed with their lengths;
+    ! trailing blanks are ignored, so a blank-padded character variable and len() will do.
+    ! readheader_hdf5 (readwrite_hdf5.f90:798-850): ints = mgalx, mx1, my, mz, nparams;
+    ! reals = alp, aspect, time, fnu, cfl, dtfixed, cmassflow
+    integer(c_int) function duct_h5_read_header(filename, ints, reals, filename_len) &
+        bind(C, name="duct_h5_read_header_")
+      import :: c_int, c_double, c_char, c_size_t
+      character(kind=c_char), intent(in) :: filename(*)
+      integer(c_int), intent(out) :: ints(5)
+      real(c_double), intent(out) :: reals(7)
+      integer(c_size_t), value :: filename_len
+    end function
+
+    ! one field of read_4fields_fou_hdf5 / read_5fields_fou_hdf5 (readwrite_hdf5.f90:2-264)
+    ! into a(my, mz, ib:ib+nmodes-1)
+    integer(c_int) function duct_h5_read_field(filename, name, a, my, mz, ib, nmodes, filename_len, name_len) &
+        bind(C, name="duct_h5_read_field_")
+      import :: c_int, c_double_complex, c_char, c_size_t
+      character(kind=c_char), intent(in) :: filename(*), name(*)
+      complex(c_double_complex), intent(out) :: a(*)
+      integer(c_int), intent(in) :: my, mz, ib, nmodes
+      integer(c_size_t), value :: filename_len, name_len
+    end function
+
+    ! a numeric dataset (header scalars, bcy, bcz) of exactly count elements
+    integer(c_int) function duct_h5_read_doubles(filename, name, out, count, filename_len, name_len) &
+        bind(C, name="duct_h5_read_doubles_")
+      import :: c_int, c_double, c_char, c_size_t, c_long_long
+      character(kind=c_char), intent(in) :: filename(*), name(*)
+      real(c_double), intent(out) :: out(*)
+      integer(c_long_long), intent(in) :: count
+      integer(c_size_t), value :: filename_len, name_len
+    end function
+
+    ! a character dataset (boundy1 ... boundz2)
+    integer(c_int) function duct_h5_read_chars(filename, name, out, capacity, length, filename_len, name_len) &
+        bind(C, name="duct_h5_read_chars_")
+      import :: c_int, c_char, c_size_t
+      character(kind=c_char), intent(in) :: filename(*), name(*)
+      character(kind=c_char), intent(out) :: out(*)
+      integer(c_int), intent(in) :: capacity
+      integer(c_int), intent(out) :: length
+      integer(c_size_t), value :: filename_len, name_len
     end function
 
     ! mixed-precision refined variant of duct_helm3d_batch_dev

@@ -316,6 +316,47 @@ int duct_set_overlap_(const int *on);
  * does not fold (n < 65, or eigenvectors without a clean parity split).     */
 int duct_fold_size_(const char *bound, int *NH, size_t bound_len);
 
+/* ---- Fourier-space field files (HDF5) ---------------------------------- */
+/* Host code: these read uvwp_NNNN.h5 / uvwpt_NNNN.h5 files into host arrays
+ * and need no device (nor duct_init_).  The library parses the HDF5 format
+ * itself (superblock versions 0-3; contiguous or compact datasets in the root
+ * group, either byte order); chunked or filtered datasets and dense link
+ * storage are DUCT_ERR_ARG with a message.  Character arguments are Fortran
+ * strings: trailing blanks are ignored.                                      */
+
+/* Header of a field file (readheader_hdf5, src/readwrite_hdf5.f90:798-850):
+ * ints[5] = {mgalx, mx1, my, mz, nparams}; reals[7] = {alp, aspect, time,
+ * fnu, cfl, dtfixed, cmassflow}.  A file without 'mx1' gives
+ * 2*(mgalx/3)/2 - 1, as the reference does (:824-826); a missing optional
+ * entry is NaN (reals) or -1 (nparams).                                      */
+int duct_h5_read_header_(const char *filename, int *ints, double *reals, size_t filename_len);
+
+/* One field ('u', 'v', 'w', 'p' or 't') into a host complex(8) array
+ * out(my, mz, nmodes) holding the modes ib .. ib+nmodes-1 (replaces the body
+ * of read_4fields_fou_hdf5 / read_5fields_fou_hdf5 and h5load_parallel,
+ * src/readwrite_hdf5.f90:2-264).  Modes past the file's mx1 are zero.  A file
+ * of another grid (my and mz both differ) is interpolated as the reference's
+ * cheb_interp does (src/chebyshev.f90:725-765: the first 20 x 20 Chebyshev
+ * coefficients evaluated on the new grid); a file where only one of my, mz
+ * differs is DUCT_ERR_UNSUPPORTED (the reference copies a mis-shaped section
+ * there, :134-140).                                                           */
+int duct_h5_read_field_(const char *filename, const char *name, double *out, const int *my,
+                        const int *mz, const int *ib, const int *nmodes, size_t filename_len,
+                        size_t name_len);
+
+/* Any numeric dataset of the root group (header scalars, the wall data bcy(mz,2,0:mx1)
+ * and bcz(my,2,0:mx1) of a temperature file) as *count doubles in the file's
+ * order; *count must equal the dataset's size (h5read_simple_double and
+ * h5read_simple_int, src/readwrite_hdf5.f90:994-1038).                       */
+int duct_h5_read_doubles_(const char *filename, const char *name, double *out,
+                          const long long *count, size_t filename_len, size_t name_len);
+
+/* A character dataset (the wall types boundy1 ... boundz2 of a temperature
+ * file, written by h5write_simple_char, :970-992) into out[0 .. *length-1];
+ * at most *capacity characters.                                              */
+int duct_h5_read_chars_(const char *filename, const char *name, char *out, const int *capacity,
+                        int *length, size_t filename_len, size_t name_len);
+
 /* ---- kernel timing (hipEvents on the library stream) -------------------- */
 
 /* Enable (1) / disable (0) per-launch event timing and clear the records. */

@@ -10,6 +10,7 @@ checked against the reference's arithmetic:
   cheb_forward / cheb_inverse   tcheb_fftw, src/tchebw.f:62-150 (REDFT00, nnyc = 0)
   derivyz                       src/chebyshev.f90:1-295 (first/second derivative recurrence)
   transyz                       src/chebyshev.f90:458-540
+  cheb_interp                   src/chebyshev.f90:725-765 (read_4fields_fou_hdf5 onto another grid)
   fourx_inverse / fourx_forward src/fourier.F:1-115 (fill, c2r / r2c, pack_fftw with 1/mgalx)
   derivx                        src/fourier.F:117-188
   mult                          src/mult.F:11-44
@@ -37,6 +38,27 @@ def cheb_forward(u: np.ndarray, axis: int) -> np.ndarray:
     out[..., 0] = y[..., 0] * (0.5 / N)
     out[..., -1] = y[..., -1] * (0.5 / N)
     return np.moveaxis(out, -1, axis)
+
+
+def cheb_interp(plane: np.ndarray, my: int, mz: int) -> np.ndarray:
+    """cheb_interp (chebyshev.f90:725-765) of one complex plane [z, y] of myr x mzr points onto my x mz.
+
+    transyz_plane_fou(-1) (:767-800) takes the plane to Chebyshev coefficients in y, then z; the
+    result at the new nodes y2 = -dglc(-1, 1, my) (:741-743, dglc :429-456) sums only the first
+    min(20, myr) x min(20, mzr) coefficients, each times cos(k acos(y2)) cos(l acos(z2)). The
+    small-coefficient test (:755) ends in `continue`, a no-op, so no term is skipped."""
+    mzr, myr = plane.shape
+    c = cheb_forward(cheb_forward(plane, axis=1), axis=0)          # [l, k]
+    ky, kz = min(20, myr), min(20, mzr)
+    pi = 4.0 * np.arctan(1.0)
+
+    def nodes(n):
+        x = -np.cos(np.arange(n) * (pi / (n - 1)))                 # dglc, then y2 = -x
+        return -x
+
+    ty = np.cos(np.outer(np.arange(ky), np.arccos(nodes(my))))    # [k, i]
+    tz = np.cos(np.outer(np.arange(kz), np.arccos(nodes(mz))))    # [l, j]
+    return tz.T @ c[:kz, :ky] @ ty                                  # [j, i]
 
 
 def cheb_inverse(c: np.ndarray, axis: int) -> np.ndarray:

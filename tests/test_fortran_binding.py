@@ -1,6 +1,7 @@
 """The Fortran binding (duct_amd/fortran/duct_hip.f90) compiles with the image's
 flang and a program calling every bound entry point links against
-libduct_hip.so - CPU only (nothing is run)."""
+libduct_hip.so - CPU only.  The device entry points are only linked; the field-file
+reader, which is host code, is also run from Fortran on a fixture."""
 import os
 import shutil
 import subprocess
@@ -19,7 +20,9 @@ program use_duct
   complex(c_double_complex) :: rhs(m, m, nm), sol(m, m, nm)
   real(c_double) :: P(m-2, m-2), Pinv(m-2, m-2), lam(m-2), DC(m, m), AM1(4, 4), U(m, m)
   integer(c_size_t) :: r(1), s(1)
-  real(c_double) :: alams(1)
+  real(c_double) :: alams(1), rh(7)
+  integer(c_int) :: ih(5), nw
+  character(kind=c_char) :: wall(8)
   rhs = 0; P = 0; Pinv = 0; lam = 0; DC = 0; AM1 = 0; U = 0; r = 0; s = 0; alams = 0
   call duct_check(duct_init(0_c_int), 'init')
   call duct_check(duct_setup('d', m, P, Pinv, lam, DC, AM1, 1_c_int, 0.0_c_double, 1_c_size_t), 'pseudomat')
@@ -50,6 +53,10 @@ program use_duct
   call duct_check(duct_setup_shen('d', m, P, P, P, P, lam, lam, lam, lam, 1_c_size_t), 'preshen')
   call duct_check(duct_helm3d_shen_dev(r(1), s(1), m, nm, 0_c_int, 1.0_c_double, 2.0_c_double, 'd', 1_c_size_t), &
        'shen')
+  call duct_check(duct_h5_read_header('f.h5', ih, rh, 4_c_size_t), 'h5 header')
+  call duct_check(duct_h5_read_field('f.h5', 'u', rhs, m, m, 0_c_int, nm, 4_c_size_t, 1_c_size_t), 'h5 field')
+  call duct_check(duct_h5_read_doubles('f.h5', 'bcy', U, 4_c_long_long, 4_c_size_t, 3_c_size_t), 'h5 bcy')
+  call duct_check(duct_h5_read_chars('f.h5', 'boundy1', wall, 8_c_int, nw, 4_c_size_t, 7_c_size_t), 'h5 wall')
   call duct_check(duct_shutdown(), 'shutdown')
 end program use_duct
 """
@@ -70,3 +77,61 @@ def test_fortran_module_compiles_and_links(tmp_path):
                     "-Wl,-rpath," + os.path.dirname(lib), "-o", str(tmp_path / "use_duct")],
                    check=True, cwd=tmp_path)
     assert (tmp_path / "use_duct").exists()
+
+
+READER = """
+program read_uvwp
+  use iso_c_binding
+  use duct_hip
+  implicit none
+  character(len=512) :: fn
+  integer(c_int) :: ih(5), nw
+  real(c_double) :: rh(7), bcy(9, 2, 0:1)
+  character(kind=c_char) :: wall(4)
+  complex(c_double_complex), allocatable :: a(:, :, :)
+  call get_command_argument(1, fn)
+  ! the whole blank-padded variable and its full length, as a Fortran caller passes it
+  call duct_check(duct_h5_read_header(fn, ih, rh, int(len(fn), c_size_t)), 'header')
+  allocate(a(ih(3), ih(4), 0:ih(2)))
+  call duct_check(duct_h5_read_field(fn, 'w', a, ih(3), ih(4), 0_c_int, ih(2) + 1_c_int, &
+                                     int(len(fn), c_size_t), 1_c_size_t), 'read w')
+  call duct_check(duct_h5_read_doubles(fn, 'bcy', bcy, 36_c_long_long, int(len(fn), c_size_t), 3_c_size_t), 'bcy')
+  call duct_check(duct_h5_read_chars(fn, 'boundz2', wall, 4_c_int, nw, int(len(fn), c_size_t), 7_c_size_t), 'wall')
+  write(*, '(5i6)') ih
+  write(*, '(3es24.16)') rh(1), rh(3), rh(4)
+  write(*, '(6es24.16)') a(1, 1, 0), a(3, 2, 1), a(ih(3), ih(4), ih(2))
+  write(*, '(2es24.16)') bcy(5, 2, 1), bcy(9, 1, 0)
+  write(*, '(i3, 1x, a1)') nw, wall(1)
+end program read_uvwp
+"""
+
+
+@pytest.mark.skipif(FC is None, reason="no Fortran compiler in this image")
+def test_fortran_reads_a_field_file(tmp_path):
+    """read_4fields_fou_hdf5's replacement called from Fortran on the temperature fixture: the field
+    w lands in a(my, mz, 0:mx1) in the reference's order (y fastest, then z, then the mode)."""
+    lib = os.path.join(ROOT, "duct_amd", "libduct_hip.so")
+    if not os.path.exists(lib):
+        pytest.skip("libduct_hip.so not built")
+    mod = os.path.join(ROOT, "duct_amd", "fortran", "duct_hip.f90")
+    prog = tmp_path / "read_uvwp.f90"
+    prog.write_text(READER)
+    subprocess.run([FC, "-c", mod, "-J", str(tmp_path), "-o", str(tmp_path / "duct_hip.o")], check=True,
+                   cwd=tmp_path)
+    subprocess.run([FC, str(prog), str(tmp_path / "duct_hip.o"), "-I", str(tmp_path), "-L",
+                    os.path.dirname(lib), "-lduct_hip", "-L/opt/rocm/lib", "-lamdhip64",
+                    "-Wl,-rpath," + os.path.dirname(lib), "-o", str(tmp_path / "read_uvwp")],
+                   check=True, cwd=tmp_path)
+    fixture = os.path.join(ROOT, "tests", "golden", "uvwpt_m9.h5")
+    out = subprocess.run([str(tmp_path / "read_uvwp"), fixture], check=True, capture_output=True, text=True,
+                         timeout=60).stdout.split("\n")
+    assert [int(x) for x in out[0].split()] == [6, 1, 9, 9, 13]
+    assert [float(x) for x in out[1].split()] == [1.0, 12.5, 0.00125]
+
+    def w(k, z, y):       # the fixture's field 2 ("w"), mode k, 0-based z and y
+        v = 3.0 + k * 0.5 + z * 0.03125 + y * 0.0009765625
+        return [v, -(v + 0.25)]
+    assert [float(x) for x in out[2].split()] == w(0, 0, 0) + w(1, 1, 2) + w(1, 8, 8)
+    # bcy(z=5, wall 2, mode 1) and bcy(z=9, wall 1, mode 0), 1-based z
+    assert [float(x) for x in out[3].split()] == [100.0 + 8.0 + 4.0 + 4 * 0.125, 100.0 + 8 * 0.125]
+    assert out[4].split() == ["1", "n"]
