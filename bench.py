@@ -66,6 +66,10 @@ def parse():
     ap.add_argument("--path", choices=["auto", "fp64", "fp64_fold_d", "fp64_fold", "mixed", "mixed_fold"],
                     default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--events-only", action="store_true",
+                    help="time only the pass with per-launch events (profiled runs: the profiler's\n                         durations then cover the launches the kernel classes come from)")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="solve the Neumann group after the Dirichlet group on one stream (profiled runs:\n                         every launch then runs alone, as in the pass the kernel classes come from)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a float64 buffer of MIB MiB (steps times): the known-size copy the HBM "
@@ -225,6 +229,7 @@ def main():
     niter = int(niter)
     torch.cuda.set_device(0)
     duct_amd.init(0)
+    duct_amd.set_overlap(not args.no_overlap)
     duct_amd.setup(m)
     fields = [workloads.rhs_field(cfg, f) for f in range(F)]
     alams = [workloads.alambda_of(cfg, f) for f in range(F)]
@@ -280,11 +285,12 @@ def main():
             steps[path]()
         torch.cuda.synchronize()
         # the step time: K steps with nothing recorded between the launches
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            steps[path]()
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
+        if not args.events_only:
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                steps[path]()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
         # the kernel classes: the same K steps again, with the library's hipEvents around every
         # launch (on its stream); the events add ~0.1 ms of gaps per step, so they stay out of dt
         with duct_amd.KernelTimer() as kt:
@@ -293,6 +299,8 @@ def main():
                 steps[path]()
             torch.cuda.synchronize()
             dt_ev = (time.perf_counter() - t0) / args.steps
+        if args.events_only:
+            dt = dt_ev
         classes = {}
         for k in kt.ms:
             if not kt.count[k]:
@@ -352,7 +360,7 @@ def main():
         "data": "synthetic (seeded Chebyshev-series right-hand sides, inputs resident in HBM)",
         "config": {"workload": "cross-plane solve stage: %d^2 CGL x %d modes x %d fields (u,v,w Dirichlet "
                                "alambda=1000; p Neumann alambda=0)" % (m, M, F),
-                   "m": m, "nmodes": M, "nfields": F, "path": chosen,
+                   "m": m, "nmodes": M, "nfields": F, "path": chosen, "overlap": not args.no_overlap,
                    "mixed": {"scheme0": s0, "scheme1": s1, "refinements": niter}},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[unit_peak] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[unit_peak],
