@@ -128,9 +128,9 @@ def test_fortran_reads_a_field_file(tmp_path):
     assert [int(x) for x in out[0].split()] == [6, 1, 9, 9, 13]
     assert [float(x) for x in out[1].split()] == [1.0, 12.5, 0.00125]
 
-    def w(k, z, y):       # the fixture's field 2 ("w"), mode k, 0-based z and y
+    def w(k, z, y):       # the fixture's field 2 ("w"), mode k, 0-based z and y; mode 0 is real
         v = 3.0 + k * 0.5 + z * 0.03125 + y * 0.0009765625
-        return [v, -(v + 0.25)]
+        return [v, -(v + 0.25) if k else 0.0]
     assert [float(x) for x in out[2].split()] == w(0, 0, 0) + w(1, 1, 2) + w(1, 8, 8)
     # bcy(z=5, wall 2, mode 1) and bcy(z=9, wall 1, mode 0), 1-based z
     assert [float(x) for x in out[3].split()] == [100.0 + 8.0 + 4.0 + 4 * 0.125, 100.0 + 8 * 0.125]

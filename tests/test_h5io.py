@@ -33,7 +33,7 @@ def exact_field(f, nm, m):
     z = np.arange(m)[None, :, None]
     y = np.arange(m)[None, None, :]
     v = (f + 1) * 1.0 + k * 0.5 + z * 0.03125 + y * 0.0009765625
-    return v - 1j * (v + 0.25)
+    return v - 1j * np.where(k == 0, 0.0, v + 0.25)        # mode 0 real
 
 
 def cheb(n, x):
@@ -42,7 +42,7 @@ def cheb(n, x):
 
 def poly(f, k, part, Y, Z, truncated):
     """The generator's poly_value; truncated drops the terms of degree >= 20 in y or z."""
-    s = (f + 1) * (k + 1) * (-0.5 if part else 1.0)
+    s = (f + 1) * (k + 1) * ((-0.5 if k else 0.0) if part else 1.0)      # mode 0 real
     v = (1.0 + 0.5 * cheb(1, Y) - 0.25 * cheb(2, Z) + 0.125 * cheb(3, Y) * cheb(1, Z)
          + 0.0625 * cheb(4, Y) * cheb(4, Z) + 0.03125 * cheb(19, Y) * cheb(2, Z))
     if not truncated:
