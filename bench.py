@@ -51,7 +51,7 @@ PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15}   # MI355X dense matrix
 # HBM bytes per launch of a kernel class at a workload (m, nmodes, nfields), from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters over a plain 1 GiB copy
 # (profiles/k_projf_f64_fold_257x256x4_pmc.md: fetch counted 0.5 per byte, write 1.0).
-TRAFFIC = {("fold", 257, 256, 4): 1116.3e6}
+TRAFFIC = {("fold", 257, 256, 4): 1116.4e6}
 TOL = 1e-10
 
 
