@@ -123,6 +123,46 @@ def test_config0_and_config1_fields(dev, oracle_mod):
         oracle_mod.use_blas(False)
 
 
+@pytest.mark.parametrize("fold", [0, 2])
+def test_batch_overlap_bit_identical(dev, oracle_mod, fold):
+    """A batch holding both boundary types runs its Neumann group on the library's second stream
+    (duct_set_overlap_, on by default), forked from and joined back into the caller's stream. The
+    solutions equal the serial order's to the bit. The check uses a caller's non-default stream,
+    interleaved bounds "dndn" with the k = 0 Poisson planes (gauge), and a copy enqueued on the
+    caller's stream right after the call, which must see the Neumann results: the join."""
+    import torch
+    m, M, ib = 129, 8, 0
+    ob = ensure(dev, oracle_mod, m)
+    fields = [random_field(m, M, 500 + i) for i in range(4)]
+    bounds = "dndn"
+    alams = [1000.0, 0.0, 250.0, 0.0]
+    rhs = [torch.from_numpy(x).cuda() for x in fields]
+    out = {}
+    try:
+        dev.set_fold(fold)
+        for ov in (False, True, False, True):
+            dev.set_overlap(ov)
+            s = torch.cuda.Stream()
+            sol = [torch.zeros_like(r) for r in rhs]
+            with torch.cuda.stream(s):
+                dev.helm3d_batch(rhs, sol, bounds, alams, alp=1.0, ib=ib)
+                copy = [x.clone() for x in sol]          # on the caller's stream, after the join
+            s.synchronize()
+            out.setdefault(ov, []).append([c.cpu().numpy() for c in copy])
+    finally:
+        dev.set_overlap(True)
+        dev.set_fold(0)
+    for ov in (False, True):
+        for run in out[ov]:
+            for f in range(4):
+                assert np.array_equal(run[f], out[False][0][f]), (ov, f)
+    for f in (0, 1):
+        ref, _ = oracle_mod.helm3d(ob, fields[f], bounds[f], alams[f], alp=1.0, ib=ib, nthreads=8)
+        e = relerr(out[True][0][f], ref)
+        print("fold %d overlap field %d (%s) rel.err %.3e" % (fold, f, bounds[f], e), flush=True)
+        assert e <= (TOL if fold == 0 else 1e-10)
+
+
 def test_modes_offset_alp_and_inplace(dev, oracle_mod):
     import torch
     m, M = 33, 5
