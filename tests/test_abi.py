@@ -3,6 +3,8 @@ header declares, and rejects bad arguments before touching the device."""
 import ctypes
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -28,6 +30,15 @@ def test_library_exports_every_declared_symbol():
     L = _lib.load()
     for s in header_symbols():
         assert getattr(L, s) is not None
+
+
+@pytest.mark.skipif(not shutil.which("gcc"), reason="no C compiler")
+def test_header_is_plain_c(tmp_path):
+    """C callers (the reference's fortran.c style) include the header as C99."""
+    src = tmp_path / "use.c"
+    src.write_text('#include "duct_hip.h"\nint main(void) { return 0; }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                    "-I", os.path.dirname(HEADER), str(src)], check=True)
 
 
 def test_missing_library_fails_loudly(tmp_path):
