@@ -382,6 +382,16 @@ void File::read_doubles(const Dataset& d, uint64_t first, uint64_t n, double* ou
         bad(path_, "unsupported element size " + std::to_string(d.size));
     if (first > d.count() || n > d.count() - first) bad(path_, "element range outside the dataset");
     const int es = d.size;
+    if (d.kind == Kind::Float && es == 8) {      // the fields: straight into out, swapped in place
+        const uint64_t off = first * 8;
+        if (d.compact) std::memcpy(out, d.inline_data.data() + off, (size_t)(n * 8));
+        else pread_exact(d.addr + off, out, (size_t)(n * 8));
+        if (d.big_endian) {
+            uint64_t* u = reinterpret_cast<uint64_t*>(out);
+            for (uint64_t i = 0; i < n; ++i) u[i] = __builtin_bswap64(u[i]);
+        }
+        return;
+    }
     constexpr uint64_t kBlock = 1 << 17;         // elements per read
     std::vector<uint8_t> buf((size_t)std::min(n, kBlock) * es);
     for (uint64_t done = 0; done < n;) {
@@ -433,22 +443,36 @@ void cheb_interp(int myr, int mzr, const double* in, int my, int mz, double* out
 {
     const int ky = std::min(20, myr), kz = std::min(20, mzr);
     const double pi = 4.0 * std::atan(1.0);
-    // cos(pi j k / N) from (j k) mod 2N, so large j k lose nothing to the argument
-    auto dct_cos = [pi](long jk, int n) { return std::cos(pi * (double)(jk % (2L * n)) / n); };
-    // y: c1[z][k][p] = s_k (f_0 + (-1)^k f_N + 2 sum_{0<j<N} f_j cos(pi j k / N))
     const int ny = myr - 1, nz = mzr - 1;
+    // cos(pi j k / N) for j <= N, k < K, from (j k) mod 2N so large j k lose nothing to the argument
+    auto dct_table = [pi](int nn, int kmax) {
+        std::vector<double> t((size_t)(nn + 1) * kmax);
+        for (int j = 0; j <= nn; ++j)
+            for (int k = 0; k < kmax; ++k) t[(size_t)j * kmax + k] = std::cos(pi * (double)((long)j * k % (2L * nn)) / nn);
+        return t;
+    };
+    const std::vector<double> cy = dct_table(ny, ky), cz = dct_table(nz, kz);
+    // y: c1[z][k][p] = s_k (f_0 + (-1)^k f_N + 2 sum_{0<j<N} f_j cos(pi j k / N))
     std::vector<double> c1((size_t)mzr * ky * 2), c2((size_t)kz * ky * 2);
-    for (int z = 0; z < mzr; ++z)
+    for (int z = 0; z < mzr; ++z) {
+        const double* f = in + (size_t)z * myr * 2;
+        double inner[20][2] = {};
+        for (int j = 1; j < ny; ++j) {
+            const double f0 = f[(size_t)j * 2], f1 = f[(size_t)j * 2 + 1];
+            const double* c = &cy[(size_t)j * ky];
+            for (int k = 0; k < ky; ++k) {
+                inner[k][0] += f0 * c[k];
+                inner[k][1] += f1 * c[k];
+            }
+        }
         for (int k = 0; k < ky; ++k) {
             const double s = (k == 0 || k == ny) ? 0.5 / ny : 1.0 / ny;
             for (int p = 0; p < 2; ++p) {
-                const double* f = in + (size_t)z * myr * 2 + p;
-                double acc = f[0] + ((k & 1) ? -f[(size_t)ny * 2] : f[(size_t)ny * 2]);
-                double inner = 0.0;
-                for (int j = 1; j < ny; ++j) inner += f[(size_t)j * 2] * dct_cos((long)j * k, ny);
-                c1[((size_t)z * ky + k) * 2 + p] = s * (acc + 2.0 * inner);
+                const double acc = f[p] + ((k & 1) ? -f[(size_t)ny * 2 + p] : f[(size_t)ny * 2 + p]);
+                c1[((size_t)z * ky + k) * 2 + p] = s * (acc + 2.0 * inner[k][p]);
             }
         }
+    }
     // z: c2[l][k][p], the same transform along z of the first ky y-coefficients
     for (int l = 0; l < kz; ++l) {
         const double s = (l == 0 || l == nz) ? 0.5 / nz : 1.0 / nz;
@@ -457,7 +481,7 @@ void cheb_interp(int myr, int mzr, const double* in, int my, int mz, double* out
                 auto g = [&](int z) { return c1[((size_t)z * ky + k) * 2 + p]; };
                 double acc = g(0) + ((l & 1) ? -g(nz) : g(nz));
                 double inner = 0.0;
-                for (int z = 1; z < nz; ++z) inner += g(z) * dct_cos((long)z * l, nz);
+                for (int z = 1; z < nz; ++z) inner += g(z) * cz[(size_t)z * kz + l];
                 c2[((size_t)l * ky + k) * 2 + p] = s * (acc + 2.0 * inner);
             }
     }
