@@ -60,7 +60,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2,
+                    help="workload (duct_amd.workloads.CONFIGS): 0 = 33^2 x 16 x 1 field, 1 = 129^2 x 64 x 4, "
+                         "2 = 257^2 x 256 x 4 (default)")
     ap.add_argument("--mixed", default="bf16x6,bf16x6,1",
                     help="scheme0,scheme1,niter of the refined solve")
     ap.add_argument("--path", choices=["auto", "fp64", "fp64_fold_d", "fp64_fold", "mixed", "mixed_fold"],
@@ -124,7 +126,7 @@ def c3_bench(steps, warmup):
     import torch
     import duct_amd
     from duct_amd import step, workloads
-    cfg = workloads.CONFIGS[3]
+    cfg = workloads.STEP_129
     torch.cuda.set_device(0)
     duct_amd.init(0)
     duct_amd.setup(cfg.m)
@@ -145,7 +147,7 @@ def c3_bench(steps, warmup):
                                  for k in kt.ms if kt.count[k]}}
     # the widened solves at the same 129^2 x 64-mode size: mixed-boundary (temperature) and Shen-Legendre
     from duct_amd import shen, temperature, workloads as W
-    rhs = torch.from_numpy(W.rhs_field(workloads.CONFIGS[1], 0)).cuda()
+    rhs = torch.from_numpy(W.rhs_field(workloads.SOLVE_129, 0)).cuda()
     sol = torch.empty_like(rhs)
     mm, MM = rhs.shape[-1], rhs.shape[0]
     bc = torch.zeros((MM, 2, mm), dtype=torch.float64, device="cuda")

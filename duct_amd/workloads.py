@@ -1,17 +1,17 @@
-"""Seeded synthetic right-hand sides for the benchmark configurations.
+"""Seeded synthetic right-hand sides for the benchmark workloads.
 
-Generator (the same for every configuration):
-  rng = numpy.random.Generator(PCG64(seed)), seed = 20260000 + 100*config + field
-  (fields u=0, v=1, w=2, p=3; config 0 = 33^2 x 16 CPU case, 1 = 129^2 x 64,
-  2 = 257^2 x 256, 3 = pseudo-spectral step, 4 = adjoint);
+Generator (the same for every workload):
+  rng = numpy.random.Generator(PCG64(seed)), seed = 20260000 + 100*index + field
+  (fields u=0, v=1, w=2, p=3; index 0 = CPU_33, the 33^2 x 16 CPU case, 1 = SOLVE_129,
+  2 = SOLVE_257, 3 = STEP_129, the pseudo-spectral step, 4 = ADJOINT_129);
   g1, g2 ~ N(0,1) of shape (M, N+1, N+1) drawn in the order (mode, q, p);
-  a_pq^(k) = (g1 + i g2) exp(-0.15 (p+q)) [times exp(-0.2 |k|) for config 3];
+  a_pq^(k) = (g1 + i g2) exp(-0.15 (p+q)) [times exp(-0.2 |k|) for STEP_129];
   f_k(y_j, z_l) = sum_pq a_pq T_p(y_j) T_q(z_l) at the CGL nodes y_j = -cos(pi j/N)
   (pseudomat.f90:83-85); the k=0 mode keeps only its real part.
 Dirichlet fields get their boundary overwritten with 0 (helm3d.f90:48-60 does
 the same); Neumann solves read only the interior.
 
-Config 3's velocity (duct_velocity) is the laminar square-duct profile plus a
+STEP_129's velocity (duct_velocity) is the laminar square-duct profile plus a
 divergence-free perturbation curl(A), A_c = (1-y^2)^2 (1-z^2)^2 h_c with h_c a
 seeded Chebyshev series, coefficients (g1 + i g2) exp(-0.2 (p + q + |k|)),
 scaled to 5% of the laminar maximum; seeds seed_for(3, c) for A_x, A_y, A_z.
@@ -45,6 +45,12 @@ CONFIGS = {
     3: Config(3, 129, 64, 4, mode_decay=0.2),
     4: Config(4, 129, 64, 4),
 }
+# The workloads by what they are (the index is only the seed key of seed_for):
+CPU_33 = CONFIGS[0]        # 33^2 CGL x 16 modes, one Dirichlet field: the CPU-runnable case
+SOLVE_129 = CONFIGS[1]     # 129^2 x 64 modes x u, v, w, p
+SOLVE_257 = CONFIGS[2]     # 257^2 x 256 modes x u, v, w, p: the benchmarked solve stage
+STEP_129 = CONFIGS[3]      # 129^2 x 64 modes, mgalx 192: the pseudo-spectral step
+ADJOINT_129 = CONFIGS[4]   # 129^2 x 64 modes: the adjoint (gradient) checks
 
 
 def cheb_nodes_matrix(m: int, deg: int) -> np.ndarray:
@@ -57,8 +63,8 @@ def cheb_nodes_matrix(m: int, deg: int) -> np.ndarray:
     return sign * np.cos(np.pi * phase / N)
 
 
-def seed_for(config: int, field: int) -> int:
-    return 20260000 + 100 * int(config) + int(field)
+def seed_for(index: int, field: int) -> int:
+    return 20260000 + 100 * int(index) + int(field)
 
 
 def rhs_field(cfg: Config, field: int, m: int | None = None, nmodes: int | None = None) -> np.ndarray:
@@ -109,7 +115,7 @@ def laminar_duct(m: int) -> np.ndarray:
 
 def duct_velocity(cfg: Config, m: int | None = None, nmodes: int | None = None, amp: float = 0.05,
                   kdeg: int = 24):
-    """u, v, w complex128 (nmodes, m, m) [mode, z, y] of the pseudo-spectral step (config 3)."""
+    """u, v, w complex128 (nmodes, m, m) [mode, z, y] of the pseudo-spectral step (STEP_129)."""
     from numpy.polynomial import chebyshev as C
     m = cfg.m if m is None else m
     M = cfg.nmodes if nmodes is None else nmodes

@@ -154,7 +154,7 @@ def test_config4_step_adjoint_identity(dev):
     """129^2 x 64 modes, FP64: <step(f), g> = <f, step^T(g)> for the full step."""
     import torch
     from duct_amd import adjoint, workloads
-    cfg = workloads.CONFIGS[4]
+    cfg = workloads.ADJOINT_129
     ensure(dev, cfg.m)
     f3 = [workloads.rhs_field(cfg, i) for i in range(3)]
     g3 = [rnd((cfg.nmodes, cfg.m, cfg.m), 40 + i) for i in range(3)]
@@ -174,7 +174,7 @@ def test_config4_mixed_forward_gradient(dev):
     (bf16x6, one refinement) against the same gradient through the FP64 forward."""
     import torch
     from duct_amd import adjoint, workloads
-    cfg = workloads.CONFIGS[4]
+    cfg = workloads.ADJOINT_129
     m, M = 257, 128
     ensure(dev, m)
     f3 = [workloads.rhs_field(cfg, i, m=m, nmodes=M) for i in range(3)]

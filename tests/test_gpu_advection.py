@@ -105,7 +105,7 @@ def test_config3_step(dev, oracle_mod):
     import torch
     from duct_amd import step, workloads
     from oracle import spectral as S
-    cfg = workloads.CONFIGS[3]
+    cfg = workloads.STEP_129
     m, M, mgalx, nu, dt = cfg.m, cfg.nmodes, 192, 0.01, 0.1
     ensure(dev, m)
     vel = workloads.duct_velocity(cfg)

@@ -90,7 +90,7 @@ def test_fold_config2_batch(dev, oracle_mod):
     folded, p unfolded; sampled modes against the oracle."""
     import torch
     from duct_amd import workloads
-    cfg = workloads.CONFIGS[2]
+    cfg = workloads.SOLVE_257
     ob = ensure(dev, oracle_mod, cfg.m)
     fields = [workloads.rhs_field(cfg, f) for f in range(4)]
     rhs = [torch.from_numpy(x).cuda() for x in fields]
@@ -150,7 +150,7 @@ def test_fold_all_config2_batch(dev, oracle_mod):
     """Config 2 with mode 2 (all four fields folded): <= 1e-10 on sampled modes."""
     import torch
     from duct_amd import workloads
-    cfg = workloads.CONFIGS[2]
+    cfg = workloads.SOLVE_257
     ob = ensure(dev, oracle_mod, cfg.m)
     fields = [workloads.rhs_field(cfg, f) for f in range(4)]
     rhs = [torch.from_numpy(x).cuda() for x in fields]

@@ -64,7 +64,7 @@ def test_mixed_matches_fp64_oracle(dev, oracle_mod, m, M, s0, s1, niter):
     from duct_amd import workloads
     ensure(dev, m)
     ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(m))
-    cfg = workloads.CONFIGS[2]
+    cfg = workloads.SOLVE_257
     fields = [workloads.rhs_field(cfg, f, m=m, nmodes=M) for f in range(4)]
     alams = [workloads.alambda_of(cfg, f) for f in range(4)]
     got, st = run_mixed(dev, fields, workloads.BOUNDS, alams, scheme0=s0, scheme1=s1, niter=niter)
@@ -91,7 +91,7 @@ def test_residual_kernel_matches_numpy(dev):
     from duct_amd import workloads
     m, M = 65, 5
     s = ensure(dev, m)
-    cfg = workloads.CONFIGS[2]
+    cfg = workloads.SOLVE_257
     fields = [workloads.rhs_field(cfg, f, m=m, nmodes=M) for f in (0, 3)]
     alams = [1000.0, 0.0]
     got, st = run_mixed(dev, fields, "dn", alams, scheme0="bf16x3", scheme1="bf16x3", niter=0)
@@ -134,7 +134,7 @@ def test_mixed_full_size_config2(dev, oracle_mod):
     """257^2 x 256 modes x 4 fields, bf16x6 solve + one bf16x6 refinement (the bench default): <= 1e-10 on sampled modes."""
     import torch
     from duct_amd import workloads
-    cfg = workloads.CONFIGS[2]
+    cfg = workloads.SOLVE_257
     ensure(dev, cfg.m)
     ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(cfg.m))
     fields = [workloads.rhs_field(cfg, f) for f in range(4)]
@@ -156,7 +156,7 @@ def test_fold16_full_size_config2(dev, oracle_mod):
     """257^2 x 256 modes x 4 fields, the folded fused solve with one refinement (scheme 5):
     <= 1e-10 on sampled modes, the low Poisson modes included."""
     from duct_amd import workloads
-    cfg = workloads.CONFIGS[2]
+    cfg = workloads.SOLVE_257
     ensure(dev, cfg.m)
     ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(cfg.m))
     fields = [workloads.rhs_field(cfg, f) for f in range(4)]

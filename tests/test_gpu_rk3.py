@@ -40,7 +40,7 @@ def initial_state(m, M, seed):
     previous advection term."""
     from duct_amd import workloads
     from oracle import spectral as S
-    cfg = workloads.CONFIGS[3]
+    cfg = workloads.STEP_129
     u, v, w = workloads.duct_velocity(cfg, m=m, nmodes=M)
     rng = np.random.default_rng(seed)
     x = -np.cos(np.pi * np.arange(m) / (m - 1))

@@ -98,14 +98,13 @@ def test_helm3d_matches_oracle(dev, oracle_mod, m, M, bound):
     assert not got[0].imag.any()                   # Im not solved at k = 0
 
 
-def test_config0_and_config1_fields(dev, oracle_mod):
+def test_cpu_case_and_129_batch_fields(dev, oracle_mod):
     """The CPU-reference case (33^2 x 16, alpha=1000) and the 129^2 x 64 x 4 batch."""
     import torch
     from duct_amd import workloads
     oracle_mod.use_blas(True)
     try:
-        for ci in (0, 1):
-            cfg = workloads.CONFIGS[ci]
+        for cfg in (workloads.CPU_33, workloads.SOLVE_129):
             ob = ensure(dev, oracle_mod, cfg.m)
             fields = [workloads.rhs_field(cfg, f) for f in range(cfg.nfields)]
             rhs = [torch.from_numpy(x).cuda() for x in fields]
@@ -117,7 +116,7 @@ def test_config0_and_config1_fields(dev, oracle_mod):
             for f in range(cfg.nfields):
                 ref, _ = oracle_mod.helm3d(ob, fields[f], bounds[f], alams[f], alp=cfg.alp, ib=0, nthreads=8)
                 e = relerr(sol[f].cpu().numpy(), ref)
-                print("config %d field %s rel.err %.3e" % (ci, workloads.FIELDS[f], e), flush=True)
+                print("%d^2 x %d field %s rel.err %.3e" % (cfg.m, cfg.nmodes, workloads.FIELDS[f], e), flush=True)
                 assert e <= TOL
     finally:
         oracle_mod.use_blas(False)
@@ -242,7 +241,7 @@ def test_full_size_config2(dev, oracle_mod):
     """257^2 x 256 modes x 4 fields: per-plane FP64 residual and sampled oracle parity."""
     import torch
     from duct_amd import workloads
-    cfg = workloads.CONFIGS[2]
+    cfg = workloads.SOLVE_257
     m, M = cfg.m, cfg.nmodes
     ob = ensure(dev, oracle_mod, m)
     s = dev.solver._state["setup"]

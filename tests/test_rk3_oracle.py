@@ -103,7 +103,7 @@ def test_projection_removes_interior_divergence(oracle_mod):
     from duct_amd import workloads
     m, M, mgalx, fnu, dt, alp = 17, 16, 48, 0.01, 0.05, 1.0
     helm3d = _helm(oracle_mod, m, alp)
-    cfg = workloads.CONFIGS[3]
+    cfg = workloads.STEP_129
     u, v, w = workloads.duct_velocity(cfg, m=m, nmodes=M)
     p = np.zeros_like(u)
     adv = (np.zeros_like(u),) * 3

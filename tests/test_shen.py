@@ -64,7 +64,7 @@ def test_shen_vs_collocation_dirichlet_converges(oracle_mod, lam):
 def test_shen_vs_collocation_config0(oracle_mod):
     """The CPU reference case: 33^2, alpha = 1000, 16 modes of the seeded spectrum."""
     from duct_amd import workloads
-    cfg = workloads.CONFIGS[0]
+    cfg = workloads.CPU_33
     m = cfg.m
     ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(m))
     f = workloads.rhs_field(cfg, 0)
