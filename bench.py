@@ -121,7 +121,7 @@ def _timed_twice(fn, steps, warmup):
 
 
 def c3_bench(steps, warmup):
-    """Config 3 (129^2 CGL, 64 modes, mgalx = 192): time of advection3d and of one projection
+    """The pseudo-spectral step at 129^2 CGL, 64 modes, mgalx = 192: time of advection3d and of one projection
     substep on device; the kernel classes come from the library's own hipEvents."""
     import torch
     import duct_amd

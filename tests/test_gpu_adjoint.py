@@ -1,5 +1,5 @@
 """GPU tests of the hand-written adjoints and of gradients through the
-projection step (config 4: J = 0.5 ||u||^2 of the output velocity, gradient
+projection step (129^2 x 64 modes: J = 0.5 ||u||^2 of the output velocity, gradient
 with respect to the forcing f).
 
 Pins: the FP64 adjoint identity <S f, g> = <f, S^T g> (real inner product
@@ -165,7 +165,7 @@ def test_config4_step_adjoint_identity(dev):
     rhs = sum(inner(f, b.cpu().numpy()) for f, b in zip(f3, fb))
     scale = np.sqrt(sum(np.linalg.norm(u.cpu().numpy()) ** 2 for u in u3)) * \
         np.sqrt(sum(np.linalg.norm(g) ** 2 for g in g3))
-    print("config 4 step adjoint identity: %.15e vs %.15e (scale %.3e)" % (lhs, rhs, scale), flush=True)
+    print("129^2x64 step adjoint identity: %.15e vs %.15e (scale %.3e)" % (lhs, rhs, scale), flush=True)
     assert abs(lhs - rhs) <= 1e-11 * scale
 
 
@@ -187,5 +187,5 @@ def test_config4_mixed_forward_gradient(dev):
         grads[name] = [t.grad.cpu().numpy() for t in ft]
         del ft, u, loss
     err = max(np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(grads["mixed"], grads["fp64"]))
-    print("config 4 mixed-forward gradient rel. diff vs FP64 forward: %.2e" % err, flush=True)
+    print("129^2x64 mixed-forward gradient rel. diff vs FP64 forward: %.2e" % err, flush=True)
     assert err <= 1e-9

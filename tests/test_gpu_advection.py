@@ -1,5 +1,5 @@
 """GPU tests of the advection term (duct_advection_dev_) and of the pseudo-spectral
-step (config 3: 192 x-points, 64 modes, 129^2 CGL points), against the oracle,
+step (192 x-points, 64 modes, 129^2 CGL points), against the oracle,
 which follows the reference's own route (FFTW-convention x transforms, REDFT00
 Chebyshev transforms and the derivyz recurrence, oracle/spectral.py) where the
 GPU uses dense x-DFTs on the matrix cores and collocation products.
@@ -100,7 +100,7 @@ def test_advection_aspect_and_errors(dev):
 
 
 def test_config3_step(dev, oracle_mod):
-    """Config 3: laminar duct + 5% divergence-free perturbation; advection, three Helmholtz
+    """129^2 x 64 modes, mgalx 192: laminar duct + 5% divergence-free perturbation; advection, three Helmholtz
     solves, the pressure Poisson solve and the projection, against the oracle."""
     import torch
     from duct_amd import step, workloads
@@ -119,7 +119,7 @@ def test_config3_step(dev, oracle_mod):
     radv = S.advection3d(*vel, mgalx=mgalx, alp=cfg.alp)
     for c in range(3):
         e = relerr(adv[c].cpu().numpy(), radv[c])
-        print("config 3 adv%d rel.err %.2e" % (c + 1, e), flush=True)
+        print("129^2x64 adv%d rel.err %.2e" % (c + 1, e), flush=True)
         assert e <= 1e-12
     oracle_mod.use_blas(True)
     try:
@@ -138,7 +138,7 @@ def test_config3_step(dev, oracle_mod):
     ref = (us[0] - dt * S.derivx(ph, cfg.alp), us[1] - dt * dy(ph), us[2] - dt * dz(ph))
     for c in range(3):
         e = relerr(got[c], ref[c])
-        print("config 3 step u%d rel.err %.2e" % (c + 1, e), flush=True)
+        print("129^2x64 step u%d rel.err %.2e" % (c + 1, e), flush=True)
         assert e <= 1e-10
     # the projection leaves the interior divergence the reference's own route leaves: its Poisson
     # operator D2B comes from the textbook DC (pseudomat.f90:90-110) while div3d and the gradient
@@ -149,7 +149,7 @@ def test_config3_step(dev, oracle_mod):
     dstar = d * dt
     rel = np.abs(div[:, 1:-1, 1:-1]).max() / np.abs(dstar[:, 1:-1, 1:-1]).max()
     rel_ref = np.abs(div_ref[:, 1:-1, 1:-1]).max() / np.abs(dstar[:, 1:-1, 1:-1]).max()
-    print("config 3 interior divergence after projection / before: %.2e (oracle route %.2e)" % (rel, rel_ref),
+    print("129^2x64 interior divergence after projection / before: %.2e (oracle route %.2e)" % (rel, rel_ref),
           flush=True)
     assert rel <= 1.05 * rel_ref + 1e-12 and rel_ref <= 5e-8
 

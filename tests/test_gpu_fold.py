@@ -108,7 +108,7 @@ def test_fold_config2_batch(dev, oracle_mod):
             for k0 in (0, 1, 200, 255):
                 ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
                 e = relerr(sol[f][k0:k0 + 1].cpu().numpy(), ref)
-                print("config 2 folded field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
+                print("257^2x256 folded field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
                 assert e <= TOL
     finally:
         oracle_mod.use_blas(False)
@@ -147,7 +147,7 @@ def test_fold_neumann(dev, oracle_mod, m, M, alam):
 
 
 def test_fold_all_config2_batch(dev, oracle_mod):
-    """Config 2 with mode 2 (all four fields folded): <= 1e-10 on sampled modes."""
+    """257^2 x 256 x 4 with fold mode 2 (all four fields folded): <= 1e-10 on sampled modes."""
     import torch
     from duct_amd import workloads
     cfg = workloads.SOLVE_257
@@ -168,7 +168,7 @@ def test_fold_all_config2_batch(dev, oracle_mod):
             for k0 in (0, 1, 2, 255):
                 ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
                 e = relerr(sol[f][k0:k0 + 1].cpu().numpy(), ref)
-                print("config 2 fold-all field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
+                print("257^2x256 fold-all field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
                 assert e <= 1e-10
     finally:
         oracle_mod.use_blas(False)

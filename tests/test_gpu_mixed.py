@@ -146,7 +146,7 @@ def test_mixed_full_size_config2(dev, oracle_mod):
             for k0 in (0, 1, 2, 100, 255):
                 ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
                 e = relerr(got[f][k0:k0 + 1], ref)
-                print("config 2 field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
+                print("257^2x256 field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
                 assert e <= TOL
     finally:
         oracle_mod.use_blas(False)
@@ -170,7 +170,7 @@ def test_fold16_full_size_config2(dev, oracle_mod):
             for k0 in (0, 1, 2, 7, 8, 100, 255):
                 ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
                 e = relerr(got[f][k0:k0 + 1], ref)
-                print("fold16 config 2 field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
+                print("fold16 257^2x256 field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
                 assert e <= TOL
     finally:
         oracle_mod.use_blas(False)

@@ -36,7 +36,7 @@ def oracle_helm(oracle_mod, m, alp):
 
 
 def initial_state(m, M, seed):
-    """Config-3 velocity (laminar duct + divergence-free perturbation), a smooth pressure and a
+    """The step's velocity (laminar duct + divergence-free perturbation), a smooth pressure and a
     previous advection term."""
     from duct_amd import workloads
     from oracle import spectral as S
@@ -87,7 +87,7 @@ def test_rk3_step_matches_oracle(dev, oracle_mod, m, M, mgalx):
 
 
 def test_rk3_substep_config3(dev, oracle_mod):
-    """Config 3 (129^2 CGL, 64 modes, mgalx 192), substep krk = 2 (reads the previous advection)."""
+    """129^2 CGL, 64 modes, mgalx 192: substep krk = 2 (reads the previous advection)."""
     from oracle import rk3 as R
     m, M, mgalx, fnu, dt, alp = 129, 64, 192, 0.01, 0.01, 1.0
     dev.setup(m)
@@ -100,7 +100,7 @@ def test_rk3_substep_config3(dev, oracle_mod):
     finally:
         oracle_mod.use_blas(False)
     errs = [relerr(g, r) for g, r in zip(gf + ga, list(ref[:4]) + list(ref[4]))]
-    print("config 3 krk=2 rel.err u v w p adv1..3: %s" % " ".join("%.1e" % e for e in errs), flush=True)
+    print("129^2x64 krk=2 rel.err u v w p adv1..3: %s" % " ".join("%.1e" % e for e in errs), flush=True)
     assert max(errs) <= 1e-10
 
 

@@ -263,7 +263,7 @@ def test_full_size_config2(dev, oracle_mod):
         absop = float(torch.from_numpy(np.abs(op)).sum(dim=1).max())
         scale = 2.0 * absop * float(sol[f].abs().max()) + float(rhs[f].abs().max())
         rel = float(r.abs().max()) / scale
-        print("config 2 field %s max interior residual / (2|D2B|max|u| + max|f|) = %.3e"
+        print("257^2x256 field %s max interior residual / (2|D2B|max|u| + max|f|) = %.3e"
               % (workloads.FIELDS[f], rel), flush=True)
         assert rel <= 1e-13
     oracle_mod.use_blas(True)

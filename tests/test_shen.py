@@ -85,5 +85,5 @@ def test_shen_vs_collocation_config0(oracle_mod):
             fk = fk.real
         us = shen.shen_solve(m, -np.real(fk), xl, "d") + 1j * shen.shen_solve(m, -np.imag(fk), xl, "d")
         worst = max(worst, np.max(np.abs(us - sol[k])) / np.max(np.abs(sol[k])))
-    print("config 0: Shen-Legendre vs collocation max rel. difference %.2e" % worst)
+    print("33^2x16: Shen-Legendre vs collocation max rel. difference %.2e" % worst)
     assert worst <= 5e-3
