@@ -1948,9 +1948,12 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
         const double al[1] = {0.0};
         if ((st = solve_fields(1, rhs, sol, "n", al, mm, M, 0, *alp, BND_DIR_ZERO))) return st;
     }
-    // (xi) update_uvwp: rhsphi = (a+b) dt nu lap(phi) -> h1; p += phi - rhsphi; u -= dt (a+b) grad phi
+    // (xi) update_uvwp: rhsphi = (a+b) dt nu lap(phi) -> h1; p += phi - rhsphi; u -= dt (a+b) grad phi.
+    // The y and z parts of lap(phi) and of grad(phi) in one launch: all four read phi only, and the
+    // x term and the p, u update that follow touch neither v nor w.
     const double sl = ab * *dt * nu, su = -*dt * ab;
-    if ((st = cline_run(mm, M, {{phi, ty, D2, sl, 0.0, 0, 1}, {phi, tz, D2, sl / (asp * asp), 0.0, 1, 1}})))
+    if ((st = cline_run(mm, M, {{phi, ty, D2, sl, 0.0, 0, 1}, {phi, tz, D2, sl / (asp * asp), 0.0, 1, 1},
+                                {phi, U[1], D1, su, 1.0, 0, -1}, {phi, U[2], D1, su / asp, 1.0, 1, -1}})))
         return st;
     {
         Timed t(PK_DERIV);
@@ -1960,7 +1963,7 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
         Timed t(PK_DERIV);
         HIPCHK(launch_rk3_update(P, U[0], phi, h[0], (long)mm * mm, M, 0, *alp, su, L.stream));
     }
-    return cline_run(mm, M, {{phi, U[1], D1, su, 1.0, 0, -1}, {phi, U[2], D1, su / asp, 1.0, 1, -1}});
+    return OK;
 }
 
 // The GPU-upload tails of pseudomaty_temp / pseudomatz_temp (pseudomat_temp.f90:2-321): the
