@@ -123,14 +123,15 @@ def test_cpu_case_and_129_batch_fields(dev, oracle_mod):
 
 
 @pytest.mark.parametrize("fold", [0, 2])
-def test_batch_overlap_bit_identical(dev, oracle_mod, fold):
+@pytest.mark.parametrize("m,M,ib,alp", [(129, 8, 0, 1.0), (257, 4, 3, 0.35)])
+def test_batch_overlap_bit_identical(dev, oracle_mod, fold, m, M, ib, alp):
     """A batch holding both boundary types runs its Neumann group on the library's second stream
     (duct_set_overlap_, on by default), forked from and joined back into the caller's stream. The
     solutions equal the serial order's to the bit. The check uses a caller's non-default stream,
-    interleaved bounds "dndn" with the k = 0 Poisson planes (gauge), and a copy enqueued on the
-    caller's stream right after the call, which must see the Neumann results: the join."""
+    interleaved bounds "dndn" (with the k = 0 Poisson planes and their gauge at ib = 0; at 257^2 a
+    mode offset and alp != 1), and a copy enqueued on the caller's stream right after the call,
+    which must see the Neumann results: the join."""
     import torch
-    m, M, ib = 129, 8, 0
     ob = ensure(dev, oracle_mod, m)
     fields = [random_field(m, M, 500 + i) for i in range(4)]
     bounds = "dndn"
@@ -144,7 +145,7 @@ def test_batch_overlap_bit_identical(dev, oracle_mod, fold):
             s = torch.cuda.Stream()
             sol = [torch.zeros_like(r) for r in rhs]
             with torch.cuda.stream(s):
-                dev.helm3d_batch(rhs, sol, bounds, alams, alp=1.0, ib=ib)
+                dev.helm3d_batch(rhs, sol, bounds, alams, alp=alp, ib=ib)
                 copy = [x.clone() for x in sol]          # on the caller's stream, after the join
             s.synchronize()
             out.setdefault(ov, []).append([c.cpu().numpy() for c in copy])
@@ -156,9 +157,9 @@ def test_batch_overlap_bit_identical(dev, oracle_mod, fold):
             for f in range(4):
                 assert np.array_equal(run[f], out[False][0][f]), (ov, f)
     for f in (0, 1):
-        ref, _ = oracle_mod.helm3d(ob, fields[f], bounds[f], alams[f], alp=1.0, ib=ib, nthreads=8)
+        ref, _ = oracle_mod.helm3d(ob, fields[f], bounds[f], alams[f], alp=alp, ib=ib, nthreads=8)
         e = relerr(out[True][0][f], ref)
-        print("fold %d overlap field %d (%s) rel.err %.3e" % (fold, f, bounds[f], e), flush=True)
+        print("m=%d ib=%d fold %d overlap field %d (%s) rel.err %.3e" % (m, ib, fold, f, bounds[f], e), flush=True)
         assert e <= (TOL if fold == 0 else 1e-10)
 
 
