@@ -15,12 +15,20 @@ Paths timed on the same inputs:
   mixed        the refined form of --mixed (default bf16x6,bf16x6,1): low-precision
                passes (FP32 or split-bf16 MFMA), FP64 residual, correction(s)
                accumulated into the FP64 solution
+  mixed_f32    the same with FP32-MFMA passes (variant A), --f32-iter refinements
   mixed_fold   the refined form in the folded basis (scheme fold16): fused per-quarter
                fp16 hi/lo solves, FP64 residual of the folded operator, one correction
                (two for the low Poisson modes)
+  mixed_fold_q the same with the residual's products formed exactly from int8 digits on the
+               int8 matrix cores (scheme fold16q), summed to FP64 accuracy
 The reported `value` is the fastest path whose result is within 1e-10 (relative,
-max norm, every field) of the oracle; every path's throughput, error and kernel
-times are in the line.
+max norm, every field) of the oracle, the metric's tolerance; value_path,
+value_tier and value_max_rel_err_vs_oracle say which path and how close.  The
+plain FP64 paths are also held to the FP64 bar 1e-12 (fp64_tier: the fastest of
+them that meets it); mixed_tier is the fastest refined path within 1e-10 and
+mixed_vs_fp64 its ratio to the plain FP64 solve.  Every path's throughput, error
+and kernel times are in the line.  Without the oracle (--no-cpu-baseline) nothing
+is verified and value is the plain FP64 path.
 
 Metric: mode-solves/s, a mode-solve being one (field, k_x) complex 2-D solve.
 Roofline: the chosen path's dominant kernel class, algorithmic FLOP per launch
@@ -30,8 +38,10 @@ Roofline: the chosen path's dominant kernel class, algorithmic FLOP per launch
 plane at k=0 not counted) over its mean launch time from hipEvents recorded on
 the library's stream during the timed steps.
 cpu_baseline: the oracle (C restatement of solvediag/int2d3d/helm3d, one BLAS
-dgemm per product) on the host cores over the whole workload; it also serves
-as the checker of both GPU results.
+dgemm per product) over the whole workload on the host cores this process may
+use (its CPU affinity, capped by OMP_NUM_THREADS when that is set: the box's
+share for one GPU); cores, host_cores and affinity_cores are all reported.  It
+also serves as the checker of every GPU result.
 """
 from __future__ import annotations
 
@@ -47,12 +57,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15}   # MI355X dense matrix peaks, FLOP/s
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "f16": 2.5e15,   # MI355X dense matrix peaks,
+        "i8": 5.0e15}                                                   # FLOP/s (int8: OP/s)
 # HBM bytes per launch of a kernel class at a workload (m, nmodes, nfields), from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters over a plain 1 GiB copy
 # (profiles/k_projf_f64_fold_257x256x4_pmc.md: fetch counted 0.5 per byte, write 1.0).
 TRAFFIC = {("fold", 257, 256, 4): 1116.4e6}
-TOL = 1e-10
+TOL = 1e-10        # the metric's tolerance (refined and folded paths)
+TOL_FP64 = 1e-12   # the FP64 bar (plain FP64 paths)
 
 
 def parse():
@@ -65,9 +77,13 @@ def parse():
                          "2 = 257^2 x 256 x 4 (default)")
     ap.add_argument("--mixed", default="bf16x6,bf16x6,1",
                     help="scheme0,scheme1,niter of the refined solve")
-    ap.add_argument("--path", choices=["auto", "fp64", "fp64_fold_d", "fp64_fold", "mixed", "mixed_fold"],
+    ap.add_argument("--path", choices=["auto", "fp64", "fp64_fold_d", "fp64_fold", "mixed", "mixed_f32",
+                                       "mixed_fold", "mixed_fold_q"],
                     default="auto")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--f32-iter", type=int, default=2,
+                    help="refinements of the FP32-MFMA refined solve (path mixed_f32)")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the oracle: the line is then unverified and value is the plain FP64 path")
     ap.add_argument("--events-only", action="store_true",
                     help="time only the pass with per-launch events (profiled runs: the profiler's\n                         durations then cover the launches the kernel classes come from)")
     ap.add_argument("--no-overlap", action="store_true",
@@ -193,6 +209,8 @@ def cpu_baseline(cfg, fields, alams, threads, reps=3):
         "value": reps * cfg.nfields * cfg.nmodes / t_all,
         "unit": "mode-solves/s",
         "cores": threads,
+        "host_cores": os.cpu_count(),
+        "affinity_cores": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
         "kind": "port",
         "sample": "the full workload (%d fields x %d modes, %d^2) %d times with the oracle (C restatement "
                   "of helm3d/int2d3d/solvediag, %s), OpenMP over modes on %d threads; %.2f s wall, "
@@ -237,7 +255,11 @@ def main():
     alams = [workloads.alambda_of(cfg, f) for f in range(F)]
     bounds = workloads.BOUNDS[:F]
     rhs = [torch.from_numpy(x).cuda() for x in fields]
-    paths = ("fp64", "fp64_fold_d", "fp64_fold", "mixed", "mixed_fold")
+    # path -> (tier, bar): the FP64 tier is held to the FP64 bar; the folded FP64 solve and the
+    # refined solves to the metric's 1e-10
+    paths = {"fp64": ("fp64", TOL_FP64), "fp64_fold_d": ("fp64", TOL_FP64), "fp64_fold": ("fp64 folded", TOL),
+             "mixed": ("mixed", TOL), "mixed_f32": ("mixed", TOL), "mixed_fold": ("mixed", TOL),
+             "mixed_fold_q": ("mixed", TOL)}
     sol = {p: [torch.zeros_like(r) for r in rhs] for p in paths}
 
     def fp64_step(path, fold):
@@ -246,39 +268,43 @@ def main():
             duct_amd.helm3d_batch(rhs, sol[path], bounds, alams, alp=cfg.alp, ib=0)
         return run
 
-    steps = {
-        "fp64": fp64_step("fp64", 0),
-        "fp64_fold_d": fp64_step("fp64_fold_d", 1),
-        "fp64_fold": fp64_step("fp64_fold", 2),
-        "mixed": lambda: duct_amd.helm3d_mixed_batch(rhs, sol["mixed"], bounds, alams, alp=cfg.alp, ib=0,
-                                                     scheme0=s0, scheme1=s1, niter=niter),
-        "mixed_fold": lambda: duct_amd.helm3d_mixed_batch(rhs, sol["mixed_fold"], bounds, alams, alp=cfg.alp,
-                                                          ib=0, scheme0="fold16", scheme1="fold16", niter=1),
-    }
+    def mixed_step(path, a, b, it):
+        return lambda: duct_amd.helm3d_mixed_batch(rhs, sol[path], bounds, alams, alp=cfg.alp, ib=0, scheme0=a,
+                                                   scheme1=b, niter=it)
+
+    schemes = {"mixed": (s0, s1, niter), "mixed_f32": ("f32", "f32", args.f32_iter),
+               "mixed_fold": ("fold16", "fold16", 1), "mixed_fold_q": ("fold16q", "fold16q", 1)}
+    steps = {"fp64": fp64_step("fp64", 0), "fp64_fold_d": fp64_step("fp64_fold_d", 1),
+             "fp64_fold": fp64_step("fp64_fold", 2)}
+    for p, (a, b, it) in schemes.items():
+        steps[p] = mixed_step(p, a, b, it)
     he, ho = (n + 1) // 2, n // 2
     planes = F * (2 * M - 1)                      # Im at k=0 not solved
     nprod = {"f32": 1, "bf16x1": 1, "bf16x3": 3, "bf16x6": 6}
+    g = 2.0 * n ** 3 * planes                    # one unfolded product of every plane
+    gf = 2.0 * (he * he + ho * ho) * n * planes  # one folded product of every plane
 
-    def flop_per_step(kind, path):
-        """Algorithmic FLOP of one step of a kernel class (the extra correction of the low
-        Poisson modes in mixed_fold is not counted)."""
-        g = 2.0 * n ** 3 * planes                  # one projection pass, one product
-        gf = 2.0 * (he * he + ho * ho) * n * planes  # one folded pass, one product
+    def work_per_step(kind, path):
+        """Executed operations of one step of a kernel class and the peak they are held to (the
+        extra correction of the low Poisson modes is not counted)."""
         if kind.startswith("proj_"):
-            return g
+            return g, "f64"
         if kind.startswith("fold_"):
-            return gf
-        if path == "mixed_fold":
-            if kind == "lp_p2p":                   # k_lpq: 4 passes x 3 products, two solves
-                return 2 * 4 * 3 * gf
-            if kind == "resid":                    # folded residual: two products
-                return 2.0 * gf
-            return 0.0                             # fold / unfold: bandwidth-bound
+            return gf, "f64"
+        if path in ("mixed_fold", "mixed_fold_q"):
+            if kind == "lp_p2p":                   # k_lpq: 4 passes x 3 fp16 products, two solves
+                return 2 * 4 * 3 * gf, "f16"
+            if kind == "resid":
+                if path == "mixed_fold":           # two folded FP64 products
+                    return 2.0 * gf, "f64"
+                return 2 * 18 * gf, "i8"           # two products x 18 int8 digit products
+            return 0.0, None                       # fold / unfold: bandwidth-bound
+        a, b, it = schemes[path]
         if kind.startswith("lp_"):
-            return g * (nprod[s0] + niter * nprod[s1])
+            return g * (nprod[a] + it * nprod[b]), ("f32" if a == "f32" else "bf16")
         if kind == "resid":
-            return 2.0 * g * niter                 # 4 n^3 per plane, one residual per correction
-        return 0.0
+            return 2.0 * g * it, "f64"             # 4 n^3 per plane, one residual per correction
+        return 0.0, None
 
     res = {}
     # --path X times only X (a profiled run then holds only X's kernels)
@@ -310,41 +336,64 @@ def main():
             classes[k] = {"launches_per_step": kt.count[k] / args.steps, "avg_us": kt.ms[k] / kt.count[k] * 1e3,
                           "ms_per_step": kt.ms[k] / args.steps}
         res[path] = {"ms_per_step": dt * 1e3, "mode_solves_per_s": F * M / dt, "kernels": classes,
-                     "ms_per_step_with_events": dt_ev * 1e3}
+                     "ms_per_step_with_events": dt_ev * 1e3, "tier": paths[path][0], "bar": paths[path][1]}
+        if path in schemes:
+            res[path]["schemes"] = {"scheme0": schemes[path][0], "scheme1": schemes[path][1],
+                                    "refinements": schemes[path][2]}
     duct_amd.set_fold(0)
 
     cpu, refs = None, None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = args.cpu_threads or (min(share, env) if env > 0 else share)
         cpu, refs = cpu_baseline(cfg, fields, alams, threads)
         for path in res:
             res[path]["max_rel_err_vs_oracle"] = max_rel_err([t.cpu().numpy() for t in sol[path]], refs)
+            res[path]["meets_bar"] = res[path]["max_rel_err_vs_oracle"] <= res[path]["bar"]
+            res[path]["meets_1e-10"] = res[path]["max_rel_err_vs_oracle"] <= TOL
 
-    ok = {p: (refs is None or res[p].get("max_rel_err_vs_oracle", 1.0) <= TOL) for p in res}
+    def fastest(cands):
+        cands = [p for p in cands if p in res]
+        return max(cands, key=lambda p: res[p]["mode_solves_per_s"]) if cands else None
+
     if args.path != "auto":
         chosen = args.path
+    elif refs is None:
+        chosen = "fp64"                            # nothing verified: the reference's own arithmetic
     else:
-        cands = [p for p in res if ok[p]] or ["fp64"]
-        chosen = max(cands, key=lambda p: res[p]["mode_solves_per_s"])
+        chosen = fastest([p for p in res if res[p]["meets_1e-10"]]) or "fp64"
 
-    # roofline of the chosen path's dominant kernel class
+    # roofline of the chosen path's dominant kernel class (executed operations per launch)
     ks = res[chosen]["kernels"]
     groups = {}
     for k, v in ks.items():
-        key = "proj" if k.startswith("proj_") else "lp" if k.startswith("lp_") else \
-            "fold" if k.startswith("fold_") else k
+        key = "proj" if k.startswith("proj_") else "lp" if k.startswith("lp_") and chosen not in (
+            "mixed_fold", "mixed_fold_q") else "fold" if k.startswith("fold_") else k
         groups.setdefault(key, []).append(k)
-    dom = max(groups, key=lambda g: sum(ks[k]["ms_per_step"] for k in groups[g]))
+    dom = max(groups, key=lambda gk: sum(ks[k]["ms_per_step"] for k in groups[gk]))
     dom_ms = sum(ks[k]["ms_per_step"] for k in groups[dom])
     dom_launches = sum(ks[k]["launches_per_step"] for k in groups[dom])
-    dom_flop = sum(flop_per_step(k, chosen) for k in groups[dom])
-    lp_peak = "bf16" if chosen == "mixed_fold" or s0 != "f32" else "f32"
-    unit_peak = {"proj": "f64", "fold": "f64", "resid": "f64", "lp": lp_peak}.get(dom, "f64")
+    dom_work = sum(work_per_step(k, chosen)[0] for k in groups[dom])
+    unit_peak = work_per_step(groups[dom][0], chosen)[1] or "f64"
     avg_launch_s = dom_ms / 1e3 / dom_launches if dom_launches else float("nan")
-    flop_per_launch = dom_flop / dom_launches if dom_launches else 0.0
-    achieved = flop_per_launch / avg_launch_s if dom_launches else 0.0
+    work_per_launch = dom_work / dom_launches if dom_launches else 0.0
+    achieved = work_per_launch / avg_launch_s if dom_launches else 0.0
+    kernel_name = {"proj": "k_proj (FP64 MFMA projection pass)",
+                   "fold": "k_projf (FP64 MFMA projection pass, even/odd folded basis)",
+                   "lp": "k_proj_lp (low-precision MFMA projection pass)",
+                   "lp_p2p": "k_lpq (fused per-quarter fp16 hi/lo solve)",
+                   "resid": {"mixed_fold": "k_resid_fold (FP64 MFMA folded residual)",
+                             "mixed_fold_q": "k_resid_i8 (int8 digit products, FP64-accurate residual)"}.get(
+                                 chosen, "k_resid (FP64 MFMA residual)")}.get(dom, dom)
 
+    fp64_best = fastest([p for p in res if res[p]["tier"] == "fp64" and res[p].get("meets_bar", refs is None)])
+    mixed_best = fastest([p for p in res if res[p]["tier"] == "mixed" and res[p].get("meets_bar", False)])
     value = res[chosen]["mode_solves_per_s"]
+    dtype = {"fp64": "f64", "fp64_fold_d": "f64", "fp64_fold": "f64",
+             "mixed_fold": "fp16 hi/lo folded, f64 residual",
+             "mixed_fold_q": "fp16 hi/lo folded, f64-accurate int8-digit residual",
+             "mixed_f32": "f32 refined, f64 residual"}.get(chosen, "%s+%s refined, f64 residual" % (s0, s1))
     out = {
         "metric": "Helmholtz mode-solves/s at 257^2 x256 modes: mixed vs FP64 @1e-10 err; %MFMA peak",
         "value": value,
@@ -356,26 +405,32 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if chosen.startswith("fp64") else
-                 ("fp16 hi/lo folded, f64 residual" if chosen == "mixed_fold" else
-                  "%s+%s refined, f64 residual" % (s0, s1)),
+        "dtype": dtype,
         "data": "synthetic (seeded Chebyshev-series right-hand sides, inputs resident in HBM)",
         "config": {"workload": "cross-plane solve stage: %d^2 CGL x %d modes x %d fields (u,v,w Dirichlet "
                                "alambda=1000; p Neumann alambda=0)" % (m, M, F),
-                   "m": m, "nmodes": M, "nfields": F, "path": chosen, "overlap": not args.no_overlap,
-                   "mixed": {"scheme0": s0, "scheme1": s1, "refinements": niter}},
+                   "m": m, "nmodes": M, "nfields": F, "path": chosen, "overlap": not args.no_overlap},
+        # what `value` is: the fastest path within the metric's tolerance of the FP64 oracle
+        "value_path": chosen,
+        "value_tier": res[chosen]["tier"],
+        "value_tolerance": TOL,
+        "value_max_rel_err_vs_oracle": res[chosen].get("max_rel_err_vs_oracle"),
+        "verified_against_oracle": refs is not None,
+        # the FP64 tier at the FP64 bar, and the refined solves against the plain FP64 solve
+        "fp64_tier": ({"path": fp64_best, "bar": TOL_FP64, "mode_solves_per_s": res[fp64_best]["mode_solves_per_s"],
+                       "max_rel_err_vs_oracle": res[fp64_best].get("max_rel_err_vs_oracle")} if fp64_best else None),
+        "mixed_tier": ({"path": mixed_best, "bar": TOL, "mode_solves_per_s": res[mixed_best]["mode_solves_per_s"],
+                        "max_rel_err_vs_oracle": res[mixed_best].get("max_rel_err_vs_oracle")} if mixed_best else None),
+        "mixed_vs_fp64": (res[mixed_best]["mode_solves_per_s"] / res["fp64"]["mode_solves_per_s"]
+                          if mixed_best and "fp64" in res else None),
+        # the stage's algorithmic work (8 n^3 per real plane, four unfolded products) over the step time;
+        # the folded and refined paths execute other amounts, which the roofline counts
+        "algorithmic_tflops": 4 * g / (res[chosen]["ms_per_step"] * 1e-3) / 1e12,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[unit_peak] / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[unit_peak],
-                     "traffic": TRAFFIC.get((dom, m, M, F)),
-                     "kernel": {"proj": "k_proj (FP64 MFMA projection pass)",
-                                "fold": "k_projf (FP64 MFMA projection pass, even/odd folded basis)",
-                                "lp": "k_proj_lp (%s MFMA projection pass)" % unit_peak,
-                                "resid": "k_resid (FP64 MFMA residual)"}.get(dom, dom),
-                     "flop_per_launch": flop_per_launch, "avg_launch_us": avg_launch_s * 1e6},
+                     "unit": "TOP/s" if unit_peak == "i8" else "TFLOP/s", "frac": achieved / PEAK[unit_peak],
+                     "traffic": TRAFFIC.get((dom, m, M, F)), "kernel": kernel_name, "peak_kind": unit_peak,
+                     "ops": "executed", "ops_per_launch": work_per_launch, "avg_launch_us": avg_launch_s * 1e6},
         "paths": res,
-        "mixed_vs_fp64": (max(res[p]["mode_solves_per_s"] for p in ("mixed", "mixed_fold") if p in res)
-                          / res["fp64"]["mode_solves_per_s"]
-                          if "fp64" in res and ("mixed" in res or "mixed_fold" in res) else None),
         "fp64_mfma_frac": None,
         "cpu_baseline": cpu,
     }
@@ -383,10 +438,9 @@ def main():
         kf = res["fp64"]["kernels"]
         proj = [k for k in kf if k.startswith("proj_")]
         t = sum(kf[k]["ms_per_step"] for k in proj) / 1e3
-        out["fp64_mfma_frac"] = (4 * 2.0 * n ** 3 * planes / t) / PEAK["f64"] if t else None
+        out["fp64_mfma_frac"] = (4 * g / t) / PEAK["f64"] if t else None
     print(json.dumps(out), flush=True)
     duct_amd.shutdown()
-
 
 if __name__ == "__main__":
     main()

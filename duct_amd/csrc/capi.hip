@@ -65,6 +65,8 @@ struct DevBasis {
     float lq_sfwd[2] = {1.f, 1.f}, lq_sbwd[2] = {1.f, 1.f};
     float* lq_lam = nullptr;
     double* sfold[2] = {nullptr, nullptr};   // centrosymmetric part of D2B, folded e / o, FP64 fragments
+    i4v* sdig[2] = {nullptr, nullptr};       // the same blocks as int8 digits (build_sdig)
+    int* sexp[2] = {nullptr, nullptr};       // and their row exponents
     double* nqf = nullptr;                   // Neumann: folded P^-1 row of the null mode
     double* npf = nullptr;                   // Neumann: folded P column of the null mode
     double pdiag = 0.0, wanted = 0.0;
@@ -135,8 +137,11 @@ struct Lib {
     int xdft_M = 0, xdft_X = 0;
     d2v* prods = nullptr;                 // the six product fields of the advection term
     size_t prods_bytes = 0;
+    hipEvent_t ev_switch = nullptr;       // orders a newly set stream after the previous one
     float* mfold = nullptr;               // folded FP32 arrays of the folded mixed solve
     size_t mfold_bytes = 0;
+    unsigned char* mdig = nullptr;        // its digit-form iterate (Ucol, Urow) and exponents
+    size_t mdig_bytes = 0;
     size_t stats_bytes = 0;
     bool prof = false;
     std::vector<ProfRec> recs;
@@ -167,7 +172,10 @@ void free_basis(DevBasis& B)
     free_ptr(B.lph_bwd); free_ptr(B.opfrag); free_ptr(B.qrow); free_ptr(B.pcol);
     free_ptr(B.AfwdT); free_ptr(B.AbwdT); free_ptr(B.ffwd); free_ptr(B.fbwd); free_ptr(B.flam);
     free_ptr(B.fprow);
-    for (int p = 0; p < 2; ++p) { free_ptr(B.lq_fwd[p]); free_ptr(B.lq_bwd[p]); free_ptr(B.sfold[p]); }
+    for (int p = 0; p < 2; ++p) {
+        free_ptr(B.lq_fwd[p]); free_ptr(B.lq_bwd[p]); free_ptr(B.sfold[p]);
+        free_ptr(B.sdig[p]); free_ptr(B.sexp[p]);
+    }
     free_ptr(B.lq_lam); free_ptr(B.nqf); free_ptr(B.npf);
     B = DevBasis{};
 }
@@ -604,21 +612,34 @@ int run_group_lp(const FieldGroup& g, int bidx, int scheme, int accumulate, doub
 constexpr double kLowModes = 64.0;
 
 int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long long* stats, const int* fidx,
-                         int nf_all)
+                         int nf_all, bool i8)
 {
     DevBasis& B = L.basis[bidx];
     const int NH = B.NH;
     const size_t arr = (size_t)g.nf * g.nmodes * 2 * 4 * NH * NH;   // floats per folded array
-    const int narr = 2 + niter + 1 + 1;                               // F, U0, D1..Dniter, Dlow, R
+    // F, R, U0 (FP64-residual form) or Flo (int8 form: U0 is in digits), D1..Dniter, Dlow
+    const int narr = 2 + 1 + niter + 1 + (i8 ? 1 : 0);                // + the rounded U0 (int8 form)
     int st = ensure_buf(L.mfold, L.mfold_bytes, arr * narr * sizeof(float));
     if (st) return st;
     float* F = L.mfold;
     float* R = L.mfold + arr;
     float* D[kMaxFoldSources + 1];
     for (int t = 0; t < niter + 2; ++t) D[t] = L.mfold + (size_t)(2 + t) * arr;   // D[0] = U0, D[niter+1] = low
+    float* Flo = i8 ? D[0] : nullptr;
+    float* U0q = i8 ? L.mfold + (size_t)(narr - 1) * arr : nullptr;
+    unsigned char* Ucol = nullptr;
+    unsigned char* Urow = nullptr;
+    int* gexp = nullptr;
+    if (i8) {
+        if ((st = ensure_buf(L.mdig, L.mdig_bytes, arr * 2 * kUDig + (arr / ((size_t)NH * NH)) * sizeof(int))))
+            return st;
+        Ucol = L.mdig;
+        Urow = L.mdig + arr * kUDig;
+        gexp = (int*)(L.mdig + arr * 2 * kUDig);
+    }
     {
         Timed t(PK_LP_C2P);
-        HIPCHK(launch_fold32(g, NH, F, L.stream));
+        HIPCHK(launch_fold32(g, NH, F, L.stream, Flo));
     }
     LpqArgs la{};
     for (int p = 0; p < 2; ++p) {
@@ -630,7 +651,32 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
     auto lpq = [&](const FieldGroup& gg, const float* in, float* out) -> int {
         Timed t(PK_LP_P2P);
         la.g = gg; la.X = in; la.Y = out;
+        la.Ucol = nullptr; la.Urow = nullptr; la.gexp = nullptr;
         HIPCHK(launch_lpq(NH, la, L.stream));
+        return OK;
+    };
+    auto nullproj = [&](const FieldGroup& gg, unsigned long long* sp, size_t off) -> int {
+        NullFoldArgs np{};
+        np.R = R + off; np.qf = B.nqf; np.pf = B.npf; np.stats = sp; np.NH = NH; np.nf = gg.nf;
+        for (int i = 0; i < gg.nf; ++i) np.slot[i] = fidx[i];
+        GaugeList gl{};
+        for (int k = 0; k < gg.nmodes; ++k)
+            for (int f = 0; f < gg.nf; ++f) {
+                if (!(std::fabs(xlambda_of(gg.alambda[f], gg.alp, gg.mode0 + k)) <= 1.0e-10)) continue;
+                for (int part = 0; part < 2; ++part) {
+                    if (part == 1 && gg.mode0 + k == 0) continue;
+                    gl.plane[gl.count++] = 2 * (k * gg.nf + f) + part;
+                    if (gl.count == kMaxGaugePlanes) {
+                        Timed t(PK_GAUGE);
+                        HIPCHK(launch_nullproj_fold(np, gl, L.stream));
+                        gl.count = 0;
+                    }
+                }
+            }
+        if (gl.count) {
+            Timed t(PK_GAUGE);
+            HIPCHK(launch_nullproj_fold(np, gl, L.stream));
+        }
         return OK;
     };
     // residual of the modes of gg (a prefix, or a suffix starting at plane offset `off`) with u = sum of D[0..nsrc)
@@ -649,37 +695,44 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
             Timed t(PK_RESID);
             HIPCHK(launch_resid_fold(ra, L.stream));
         }
-        if (bidx == 1) {
-            NullFoldArgs np{};
-            np.R = R + off; np.qf = B.nqf; np.pf = B.npf; np.stats = sp; np.NH = NH; np.nf = gg.nf;
-            for (int i = 0; i < gg.nf; ++i) np.slot[i] = fidx[i];
-            GaugeList gl{};
-            for (int k = 0; k < gg.nmodes; ++k)
-                for (int f = 0; f < gg.nf; ++f) {
-                    if (!(std::fabs(xlambda_of(gg.alambda[f], gg.alp, gg.mode0 + k)) <= 1.0e-10)) continue;
-                    for (int part = 0; part < 2; ++part) {
-                        if (part == 1 && gg.mode0 + k == 0) continue;
-                        gl.plane[gl.count++] = 2 * (k * gg.nf + f) + part;
-                        if (gl.count == kMaxGaugePlanes) {
-                            Timed t(PK_GAUGE);
-                            HIPCHK(launch_nullproj_fold(np, gl, L.stream));
-                            gl.count = 0;
-                        }
-                    }
-                }
-            if (gl.count) {
-                Timed t(PK_GAUGE);
-                HIPCHK(launch_nullproj_fold(np, gl, L.stream));
-            }
-        }
+        if (bidx == 1 && (st = nullproj(gg, sp, off))) return st;
         return OK;
     };
-    if ((st = lpq(g, F, D[0]))) return st;
-    for (int it = 0; it < niter; ++it) {
-        if ((st = resid(g, it + 1, stats ? stats + (size_t)2 * nf_all * it : nullptr, 0))) return st;
-        if ((st = lpq(g, R, D[it + 1]))) return st;
-    }
-    {
+    if (i8) {
+        // U0 in digit form, the residual on the int8 matrix cores, one correction (niter == 1)
+        {
+            Timed t(PK_LP_P2P);
+            la.g = g; la.X = F; la.Y = U0q;
+            la.Ucol = Ucol; la.Urow = Urow; la.gexp = gexp;
+            HIPCHK(launch_lpq(NH, la, L.stream));
+        }
+        ResidI8Args ra{};
+        ra.g = g;
+        ra.Fhi = F; ra.Flo = Flo;
+        ra.Ucol = Ucol; ra.Urow = Urow; ra.gexp = gexp; ra.U0 = U0q;
+        ra.R = R;
+        ra.Sdig[0] = B.sdig[0]; ra.Sdig[1] = B.sdig[1];
+        ra.Sexp[0] = B.sexp[0]; ra.Sexp[1] = B.sexp[1];
+        unsigned long long* sp = stats ? stats : nullptr;
+        ra.stats = sp;
+        for (int i = 0; i < g.nf; ++i) ra.slot[i] = fidx[i];
+        ra.neumann = bidx == 1;
+        ra.NH = NH;
+        {
+            Timed t(PK_RESID);
+            HIPCHK(launch_resid_i8(ra, L.stream));
+        }
+        if (bidx == 1 && (st = nullproj(g, sp, 0))) return st;
+        if ((st = lpq(g, R, D[1]))) return st;
+        Timed t(PK_LP_P2C);
+        const float* src[kMaxFoldSources] = {U0q, D[1], nullptr, nullptr};
+        HIPCHK(launch_unfold_sum(g, NH, src, 2, 0, L.stream));
+    } else {
+        if ((st = lpq(g, F, D[0]))) return st;
+        for (int it = 0; it < niter; ++it) {
+            if ((st = resid(g, it + 1, stats ? stats + (size_t)2 * nf_all * it : nullptr, 0))) return st;
+            if ((st = lpq(g, R, D[it + 1]))) return st;
+        }
         Timed t(PK_LP_P2C);
         const float* src[kMaxFoldSources] = {D[0], niter >= 1 ? D[1] : nullptr, niter >= 2 ? D[2] : nullptr, nullptr};
         HIPCHK(launch_unfold_sum(g, NH, src, niter + 1, 0, L.stream));
@@ -794,13 +847,16 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
     if (nf < 1 || nf > kMaxFields) return fail(ERR_ARG, "nfields=%d outside [1, %d]", nf, kMaxFields);
     int st = check_common(m, nmodes);
     if (st) return st;
-    const bool folded = scheme0 == LP_FOLD16 || scheme1 == LP_FOLD16;
+    const bool folded = scheme0 == LP_FOLD16 || scheme1 == LP_FOLD16 || scheme0 == LP_FOLD16Q ||
+                        scheme1 == LP_FOLD16Q;
+    const bool i8 = scheme0 == LP_FOLD16Q;
     if (folded && scheme0 != scheme1)
-        return fail(ERR_ARG, "scheme 5 (folded) is used for both solves or neither");
+        return fail(ERR_ARG, "schemes 5 and 6 (folded) are used for both solves or neither");
     for (int sc : {scheme0, scheme1})
-        if (sc < LP_F32 || sc > LP_FOLD16) return fail(ERR_ARG, "precision scheme %d is not one of 1..5", sc);
+        if (sc < LP_F32 || sc > LP_FOLD16Q) return fail(ERR_ARG, "precision scheme %d is not one of 1..6", sc);
     if (niter < 0 || niter > 16) return fail(ERR_ARG, "niter=%d outside [0, 16]", niter);
     if (folded && niter > 2) return fail(ERR_ARG, "the folded solve takes niter <= 2 (got %d)", niter);
+    if (i8 && niter != 1) return fail(ERR_ARG, "scheme 6 takes exactly one refinement (got niter=%d)", niter);
     const size_t field_bytes = (size_t)nmodes * m * m * sizeof(d2v);
     if ((st = ensure_buf(L.rfield, L.rfield_bytes, field_bytes * nf))) return st;
     const int nstat = stats ? (niter + 1) : 0;
@@ -831,7 +887,7 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
             if (!B.lq_ready)
                 return fail(ERR_UNSUPPORTED, "scheme 5 needs a folding basis with ceil((m-2)/2) in (32, 128] "
                             "(m=%d, bound '%c')", m, bidx ? 'n' : 'd');
-            if ((st = run_group_mixed_fold(g, bidx, niter, nstat ? L.stats : nullptr, fidx, nf))) return st;
+            if ((st = run_group_mixed_fold(g, bidx, niter, nstat ? L.stats : nullptr, fidx, nf, i8))) return st;
             continue;
         }
         if ((st = run_group_lp(g, bidx, scheme0, 0, B.wanted))) return st;
@@ -1074,8 +1130,10 @@ int duct_shutdown_(void)
     L.plan2_bytes = 0;
     if (L.ev_fork) (void)hipEventDestroy(L.ev_fork);
     if (L.ev_join) (void)hipEventDestroy(L.ev_join);
+    if (L.ev_switch) (void)hipEventDestroy(L.ev_switch);
     if (L.side) (void)hipStreamDestroy(L.side);
-    L.ev_fork = L.ev_join = nullptr;
+    L.ev_fork = L.ev_join = L.ev_switch = nullptr;
+    L.stream = nullptr;   // the caller's stream may not outlive the library state
     L.side = nullptr;
     for (auto& p : L.hfield) free_ptr(p);
     free_ptr(L.rfield);
@@ -1105,6 +1163,8 @@ int duct_shutdown_(void)
     L.prods_bytes = 0;
     free_ptr(L.mfold);
     L.mfold_bytes = 0;
+    free_ptr(L.mdig);
+    L.mdig_bytes = 0;
     L.plan_bytes = L.hfield_bytes = L.rfield_bytes = L.stats_bytes = 0;
     for (auto& r : L.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto& e : L.pool) (void)hipEventDestroy(e);
@@ -1130,7 +1190,20 @@ int duct_set_stream_(const duct_devptr_t* stream)
     if (!stream) return fail(ERR_ARG, "null stream pointer");
     int st = ensure_init();
     if (st) return st;
-    L.stream = (hipStream_t)(*stream);
+    const hipStream_t ns = (hipStream_t)(*stream);
+    if (ns == L.stream) return OK;
+    // Every call shares the library's scratch buffers (planar intermediates, folded arrays, residual
+    // fields), so work still queued on the old stream may yet read or write them: the new stream
+    // waits for it.  Under graph capture of either stream the caller orders the streams itself.
+    hipStreamCaptureStatus cold = hipStreamCaptureStatusNone, cnew = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(L.stream, &cold));
+    HIPCHK(hipStreamIsCapturing(ns, &cnew));
+    if (cold == hipStreamCaptureStatusNone && cnew == hipStreamCaptureStatusNone) {
+        if (!L.ev_switch) HIPCHK(hipEventCreateWithFlags(&L.ev_switch, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(L.ev_switch, L.stream));
+        HIPCHK(hipStreamWaitEvent(ns, L.ev_switch, 0));
+    }
+    L.stream = ns;
     return OK;
 }
 
@@ -1347,6 +1420,15 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
                         }
                     build_frag(Sb.data(), h, NH, NH, sfr.data());
                     if ((st = up(B.sfold[par], sfr.data(), sfr.size()))) return st;
+                    std::vector<i4v> sd;
+                    std::vector<int> se;
+                    build_sdig(Sb.data(), NH, sd, se);
+                    void* dv = nullptr;
+                    if ((st = upraw2(dv, sd.data(), sd.size() * sizeof(i4v)))) return st;
+                    B.sdig[par] = (i4v*)dv;
+                    dv = nullptr;
+                    if ((st = upraw2(dv, se.data(), se.size() * sizeof(int)))) return st;
+                    B.sexp[par] = (int*)dv;
                 }
                 bool ok = true;
                 if (bidx == 1) {

@@ -320,9 +320,11 @@ Dataset File::dataset(const std::string& name) const
             if (m.size < 2) bad(path_, "truncated layout message");
             const int ver = b[0];
             if (ver == 1 || ver == 2) {
+                if (b.size() < 8) bad(path_, "truncated layout message");
                 const int nd = b[1], cls = b[2];
                 size_t q = 8;
                 if (cls == 1) {
+                    if (q + (size_t)so_ > b.size()) bad(path_, "truncated layout message");
                     d.addr = le(&b[q], so_);
                     d.compact = false;
                 } else if (cls == 0) {
@@ -338,6 +340,7 @@ Dataset File::dataset(const std::string& name) const
             } else if (ver == 3 || ver == 4) {
                 const int cls = b[1];
                 if (cls == 0) {
+                    if (b.size() < 4) bad(path_, "truncated layout message");
                     const uint64_t n = le(&b[2], 2);
                     if (4 + n > b.size()) bad(path_, "truncated layout message");
                     d.inline_data.assign(b.begin() + 4, b.begin() + 4 + (long)n);

@@ -14,8 +14,10 @@ namespace duct {
 constexpr int kMaxFoldSources = 4;
 
 // fold: FP32 quarters of the interior of complex fields g.in (half-sum/half-difference
-// in both directions, /4), zero padding, Im of mode 0 zero.
-hipError_t launch_fold32(const FieldGroup& g, int NH, float* X, hipStream_t st);
+// in both directions, /4), zero padding, Im of mode 0 zero.  With Xlo non-null it also
+// receives the FP32 remainder (q - X) of every FP64 quarter value q, so that X + Xlo
+// carries q to ~2^-48.
+hipError_t launch_fold32(const FieldGroup& g, int NH, float* X, hipStream_t st, float* Xlo = nullptr);
 
 // unfold: g.out interior (= or +=) the unfolded sum of nsrc folded FP32 arrays (summed in FP64).
 hipError_t launch_unfold_sum(const FieldGroup& g, int NH, const float* const* src, int nsrc, int accumulate,
@@ -30,10 +32,19 @@ struct LpqArgs {
     const float* lam;       // [lam_e | lam_o], NH each
     const float* X;         // folded FP32 input
     float* Y;               // folded FP32 output
+    // Digit output (Ucol non-null): the solution U of each quarter is rounded to the grid
+    // 2^(g-24) Z with 2^g > max|U| / 0.49 (g per quarter, in gexp[quarter]); Y receives the
+    // rounded U (exact in FP32) and Ucol / Urow the three signed base-256 digits of
+    // Z = c1 2^16 + c2 2^8 + c3, |c| <= 127, as bytes: Ucol[quarter][d][z'][y'] and the
+    // transposed Urow[quarter][d][y'][z'] (d = 0 for c1).
+    unsigned char* Ucol;
+    unsigned char* Urow;
+    int* gexp;
     int gauge;              // index of the Neumann null mode among the even eigenvectors, -1 if none
     FieldGroup g;           // planes, xlambda per plane
 };
 hipError_t launch_lpq(int NH, const LpqArgs& a, hipStream_t st);
+constexpr int kUDig = 3;    // digits of a quantised quarter solution
 // fp16 hi/lo fragments of an NH x NH block B (row i, column k, column-major with leading dim ld);
 // returns the scale s (B * s has max magnitude in [0.5, 1)).
 float build_lpq_basis(const double* B, int ld, int NH, std::vector<unsigned short>& frag);
@@ -52,6 +63,37 @@ struct ResidFoldArgs {
     int NH;
 };
 hipError_t launch_resid_fold(const ResidFoldArgs& a, hipStream_t st);
+
+// The same folded residual with the two products S_r U and U S_c^T on the int8 matrix cores
+// (resid_i8.hip): S = 2^e(row) sum_t d_t 2^-8t with kSDig base-256 digits per row (exact to
+// 2^-57 of the row's maximum), U = the digit form that launch_lpq wrote (exact), and every
+// digit product with t + u <= kDigT formed exactly in int32, the levels summed in FP64, so that
+// the products come out with FP64-level error.  f is read as the FP32 pair Fhi + Flo of
+// launch_fold32.
+constexpr int kSDig = 7;
+constexpr int kDigT = 8;
+typedef int i4v __attribute__((ext_vector_type(4)));
+struct ResidI8Args {
+    FieldGroup g;                       // planes and xlambda
+    const float* Fhi;                   // folded f (launch_fold32 X)
+    const float* Flo;                   // its remainder (launch_fold32 Xlo)
+    const unsigned char* Ucol;          // digits of U (launch_lpq)
+    const unsigned char* Urow;
+    const int* gexp;
+    const float* U0;                    // U itself (launch_lpq's Y in digit mode, exact in FP32)
+    float* R;                           // folded FP32 residual
+    const i4v* Sdig[2];                 // S_e, S_o digits, layout of build_sdig
+    const int* Sexp[2];                 // their row exponents, NH each
+    unsigned long long* stats;          // as ResidFoldArgs
+    int slot[kMaxFields];
+    int neumann;
+    int NH;
+};
+hipError_t launch_resid_i8(const ResidI8Args& a, hipStream_t st);
+// Digits of an NH x NH block B (h x h valid, column-major, leading dimension NH) in the fragment
+// layout of the int8 products: frag[((rt kSDig + t) KC + kc) 64 + lane] holds the 16 bytes
+// d_(t+1)(rt 16 + lane%16, kc 64 + (lane/16) 16 + b), KC = NH/64; exps[row] = e.
+void build_sdig(const double* B, int NH, std::vector<i4v>& frag, std::vector<int>& exps);
 
 // Singular (Neumann, |xlambda| <= 1e-10) planes: r_ee -= (qf^T r_ee qf) pf pf^T, the null-space
 // part of the residual removed in FP64; stats = max|r| over the four quarters afterwards.

@@ -30,6 +30,29 @@ typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 typedef _Float16 h4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
+// Maximum over the 64 lanes of a wave of non-negative values: DPP row moves (quad_perm [1,0,3,2],
+// [2,3,0,1], row_half_mirror, row_mirror) leave each row's maximum in all its lanes, the four
+// rows are read with readlane.  A chain of __shfl_xor would take six LDS-crossbar round trips.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF,
+                                                                 false));
+}
+
+__device__ __forceinline__ float wave_max_nonneg(float v)
+{
+    v = fmaxf(v, dpp_f<0xB1>(v));
+    v = fmaxf(v, dpp_f<0x4E>(v));
+    v = fmaxf(v, dpp_f<0x141>(v));
+    v = fmaxf(v, dpp_f<0x140>(v));
+    const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+    const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16));
+    const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+    const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
 __device__ __forceinline__ void plane_decode(const FieldGroup& g, long fmp, int& f, int& mode)
 {
     mode = (int)(fmp / g.nf);
@@ -37,7 +60,7 @@ __device__ __forceinline__ void plane_decode(const FieldGroup& g, long fmp, int&
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fold32(FieldGroup g, int NH, float* X)
+__global__ __launch_bounds__(256) void k_fold32(FieldGroup g, int NH, float* X, float* Xlo)
 {
     const long idx = (long)blockIdx.x * 256 + threadIdx.x;
     const long per = (long)NH * NH;
@@ -63,16 +86,21 @@ __global__ __launch_bounds__(256) void k_fold32(FieldGroup g, int NH, float* X)
     for (int part = 0; part < 2; ++part) {
         const long p = 2 * fmp + part;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            X[((size_t)(p * 4 + k) * NH + zq) * NH + yq] = (part && zim) ? 0.f : (float)q[k][part];
+        for (int k = 0; k < 4; ++k) {
+            const size_t o = ((size_t)(p * 4 + k) * NH + zq) * NH + yq;
+            const double v = (part && zim) ? 0.0 : q[k][part];
+            const float hi = (float)v;
+            X[o] = hi;
+            if (Xlo) Xlo[o] = (float)(v - (double)hi);
+        }
     }
 }
 
-hipError_t launch_fold32(const FieldGroup& g, int NH, float* X, hipStream_t st)
+hipError_t launch_fold32(const FieldGroup& g, int NH, float* X, hipStream_t st, float* Xlo)
 {
     const long total = (long)g.nf * g.nmodes * NH * NH;
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fold32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, g, NH, X);
+    hipLaunchKernelGGL(k_fold32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, g, NH, X, Xlo);
     return hipGetLastError();
 }
 
@@ -167,6 +195,7 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
     constexpr int PER = NH * NH / 2048;  // float4 loads per thread
     __shared__ __attribute__((aligned(16))) _Float16 Xs[2][NH * LDX];
     __shared__ float red[8];
+    __shared__ float lam_s[2 * NH];      // [lam_e | lam_o] for the divide of pass 1
 
     const FieldGroup& g = a.g;
     const long plane = blockIdx.x >> 2;
@@ -184,6 +213,13 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
     float* Y = a.Y + qoff;
 
     if (part == 1 && g.mode0 + mode == 0) {   // Im of mode 0 is not solved (int2d3d.f90:43)
+        if (a.Ucol) {
+            for (int e = tid; e < kUDig * NH * NH / 16; e += 512) {
+                ((uint4*)(a.Ucol + qoff * kUDig))[e] = uint4{0u, 0u, 0u, 0u};
+                ((uint4*)(a.Urow + qoff * kUDig))[e] = uint4{0u, 0u, 0u, 0u};
+            }
+            if (tid == 0) a.gexp[plane * 4 + q] = 0;
+        }
         for (int e = tid; e < NH * NH / 4; e += 512) ((f4v*)Y)[e] = f4v{0.f, 0.f, 0.f, 0.f};
         return;
     }
@@ -191,9 +227,8 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
     const float xlam = (float)xl64;
     const bool gauge = a.gauge >= 0 && q == 0 && fabs(xl64) <= 1.0e-10;
 
-    auto block_max = [&](float v) -> float {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    auto block_max = [&](float v) -> float {   // v >= 0
+        v = wave_max_nonneg(v);
         if (lane == 0) red[wave] = v;
         __syncthreads();
         float mx = red[0];
@@ -221,6 +256,7 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
     };
 
     // ---- load X (column z', rows y'), scale and split
+    for (int e = tid; e < 2 * NH; e += 512) lam_s[e] = a.lam[e];   // visible after the barriers below
     float sx;
     {
         f4v xv[PER];
@@ -275,17 +311,23 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
                 ah[t] = *(const h8v*)&Xs[0][j * LDX + k];
                 al[t] = *(const h8v*)&Xs[1][j * LDX + k];
             }
+            // the two products into ac are TJ TI launches apart, not back to back
 #pragma unroll
             for (int y = 0; y < TI; ++y)
 #pragma unroll
                 for (int x = 0; x < TJ; ++x) {
                     am[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[x], bq[0][y][kc], am[x][y], 0, 0, 0);
                     ac[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[x], bq[1][y][kc], ac[x][y], 0, 0, 0);
-                    ac[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[x], bq[0][y][kc], ac[x][y], 0, 0, 0);
                 }
+#pragma unroll
+            for (int y = 0; y < TI; ++y)
+#pragma unroll
+                for (int x = 0; x < TJ; ++x)
+                    ac[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[x], bq[0][y][kc], ac[x][y], 0, 0, 0);
         }
         if (pass < 3) load_b(pass + 1);
-        // true values: (am + ac / 2048) / (sx sb), kept in am; D[j][i]: lane i + 16 (j/4), element j%4
+        // true values: (am + ac / 2048) inv, inv = 1 / (sx sb) a power of two that is applied
+        // together with the next pass's scale; D[j][i]: lane i + 16 (j/4), element j%4
         const float inv = 1.f / (sx * sb);
         float mx = 0.f;
 #pragma unroll
@@ -294,34 +336,82 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
             for (int y = 0; y < TI; ++y)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    float t = (am[x][y][e] + ac[x][y][e] * (1.f / 2048.f)) * inv;
+                    float t = am[x][y][e] + ac[x][y][e] * (1.f / 2048.f);
                     if (pass == 1) {   // G = ... ./ (lam_r(j) + lam_c(i) - xlambda), solvediag.f90:61-66, :151-159
                         const int j = j0 + x * 16 + l4 * 4 + e, i = i0 + y * 16 + l16;
-                        if (j < hh[r] && i < hh[c] && !(gauge && j == a.gauge && i == a.gauge))
-                            t = t / (a.lam[r * NH + j] + a.lam[c * NH + i] - xlam);
-                        else
-                            t = 0.f;
+                        const bool ok = j < hh[r] && i < hh[c] && !(gauge && j == a.gauge && i == a.gauge);
+                        const float den = lam_s[r * NH + j] + lam_s[c * NH + i] - xlam;
+                        t = ok ? t * __builtin_amdgcn_rcpf(den) : 0.f;   // ~1 ulp, inside the split's 2^-22
                     }
                     am[x][y][e] = t;
                     mx = fmaxf(mx, fabsf(t));
                 }
+        mx *= inv;
+        if (pass == 3) {
+#pragma unroll
+            for (int x = 0; x < TJ; ++x)
+#pragma unroll
+                for (int y = 0; y < TI; ++y) am[x][y] *= inv;
+        }
         if (pass == 3) {   // U(y' = j, z' = i) -> column i, rows j
+            if (!a.Ucol) {
+#pragma unroll
+                for (int x = 0; x < TJ; ++x)
+#pragma unroll
+                    for (int y = 0; y < TI; ++y) {
+                        const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
+                        *(f4v*)&Y[(size_t)i * NH + j] = am[x][y];
+                    }
+                break;
+            }
+            // digit form: Z = rint(U 2^(24-g)), |Z| < 0.49 2^24, as signed base-256 digits
+            const float gmx = block_max(mx);   // the barrier inside: every wave is done reading Xs
+            int ge = 0;
+            if (gmx > 0.f && gmx < 3.0e38f) (void)frexpf(gmx / 0.49f, &ge);
+            const float sc = ldexpf(1.f, 24 - ge);
+            const float isc = ldexpf(1.f, ge - 24);
+            unsigned char* Xb = (unsigned char*)&Xs[0][0];   // [d][y'][z'], kUDig NH^2 bytes
+            unsigned char* uc = a.Ucol + qoff * kUDig;
 #pragma unroll
             for (int x = 0; x < TJ; ++x)
 #pragma unroll
                 for (int y = 0; y < TI; ++y) {
                     const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
-                    *(f4v*)&Y[(size_t)i * NH + j] = am[x][y];
+                    unsigned w[kUDig] = {0u, 0u, 0u};
+                    f4v uq;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        int Z = __float2int_rn(am[x][y][e] * sc);
+                        uq[e] = (float)Z * isc;   // |Z| < 2^23: exact
+                        const int c3 = ((Z + 128) & 255) - 128;
+                        Z = (Z - c3) >> 8;
+                        const int c2 = ((Z + 128) & 255) - 128;
+                        const int c1 = (Z - c2) >> 8;
+                        const int cd[kUDig] = {c1, c2, c3};
+#pragma unroll
+                        for (int d = 0; d < kUDig; ++d) {
+                            w[d] |= (unsigned)(cd[d] & 255) << (8 * e);
+                            Xb[((size_t)d * NH + j + e) * NH + i] = (unsigned char)(cd[d] & 255);
+                        }
+                    }
+#pragma unroll
+                    for (int d = 0; d < kUDig; ++d) *(unsigned*)&uc[((size_t)d * NH + i) * NH + j] = w[d];
+                    *(f4v*)&Y[(size_t)i * NH + j] = uq;
                 }
+            __syncthreads();
+            unsigned char* ur = a.Urow + qoff * kUDig;
+            for (int e = tid; e < kUDig * NH * NH / 16; e += 512) ((uint4*)ur)[e] = ((const uint4*)Xb)[e];
+            if (tid == 0) a.gexp[plane * 4 + q] = ge;
             break;
         }
         sx = scale_of(block_max(mx));   // the barrier inside: every wave is done reading Xs
+        const float sput = inv * sx;     // a power of two
 #pragma unroll
         for (int x = 0; x < TJ; ++x)
 #pragma unroll
             for (int y = 0; y < TI; ++y) {
                 const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
-                put4(i, j, am[x][y], sx);
+                put4(i, j, am[x][y], sput);
             }
         __syncthreads();
     }

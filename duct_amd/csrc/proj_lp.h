@@ -13,6 +13,7 @@ enum LpScheme : int {
     LP_BF16X6 = 3,  // bf16, 3-term split, 6 products
     LP_BF16X1 = 4,  // plain bf16
     LP_FOLD16 = 5,  // even/odd folded, fused per-quarter solve, fp16 hi/lo split (3 products)
+    LP_FOLD16Q = 6, // LP_FOLD16 with the FP64-accurate residual formed from int8 digit products
 };
 
 struct ProjLpArgs {

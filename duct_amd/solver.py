@@ -184,7 +184,7 @@ class KernelTimer:
         return False
 
 
-SCHEMES = {"f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16x1": 4, "fold16": 5}
+SCHEMES = {"f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16x1": 4, "fold16": 5, "fold16q": 6}
 
 
 def helm3d_mixed_batch(rhs_list, sol_list, bounds: str, alambdas, alp: float = 1.0, ib: int = 0,

@@ -47,7 +47,10 @@ int duct_shutdown_(void);
 
 /* Stream for all later launches (a hipStream_t held as an integer, 0 = null
  * stream).  Replaces cuda_stream_create_/cublas_set_stream_
- * (src/mytranspose_cuda.cu:198-207). */
+ * (src/mytranspose_cuda.cu:198-207).  The library's scratch buffers are shared
+ * by all calls, so a new stream is made to wait (hipStreamWaitEvent) for the
+ * work queued on the previous one; when either stream is being captured into
+ * a graph no wait is added and the caller orders the two streams. */
 int duct_set_stream_(const duct_devptr_t *stream);
 
 /* Wait for all work the library queued on its stream. */
@@ -125,7 +128,11 @@ int duct_solvediag_(double *U, const char *bound, const double *xlambda, size_t 
  * split (3 products), FP32 intermediates, the FP64 residual of the
  * centrosymmetric part of D2B in the folded basis, one extra correction of
  * the Poisson modes with xlambda < 64 (scheme0 == scheme1 == 5, niter <= 2,
- * m with 32 < ceil((m-2)/2) <= 128).  niter in [0, 16].
+ * m with 32 < ceil((m-2)/2) <= 128), 6 = scheme 5 with the residual's two
+ * products formed exactly from int8 digits on the int8 matrix cores (seven
+ * base-256 digits of each row of the operator, three of the first solve's
+ * result rounded to 2^-24 of its quarter's maximum) and summed to FP64
+ * accuracy (scheme0 == scheme1 == 6, niter == 1).  niter in [0, 16].
  * stats (host, may be NULL): (niter+1) x nfields x 2 doubles = max|r| after
  * each step (the last one after the final correction) and max|f|.           */
 int duct_helm3d_mixed_batch_dev_(const int *nfields, const duct_devptr_t *rhs,

@@ -144,6 +144,7 @@ def test_batch_overlap_bit_identical(dev, oracle_mod, fold, m, M, ib, alp):
             dev.set_overlap(ov)
             s = torch.cuda.Stream()
             sol = [torch.zeros_like(r) for r in rhs]
+            s.wait_stream(torch.cuda.current_stream())   # the zero fill of sol comes first
             with torch.cuda.stream(s):
                 dev.helm3d_batch(rhs, sol, bounds, alams, alp=alp, ib=ib)
                 copy = [x.clone() for x in sol]          # on the caller's stream, after the join
@@ -276,3 +277,38 @@ def test_full_size_config2(dev, oracle_mod):
                 assert e <= TOL, (f, k0, e)
     finally:
         oracle_mod.use_blas(False)
+
+
+def test_stream_switch_orders_shared_scratch(dev, oracle_mod):
+    """Two batched solves queued on two different streams with no host synchronisation between
+    them: the library's scratch buffers are shared, so the second stream must wait for the first
+    (duct_set_stream_ orders a new stream after the previous one). Each result equals the same
+    solve run alone, to the bit."""
+    import torch
+    m, M = 129, 16
+    ensure(dev, oracle_mod, m)
+    fa = [random_field(m, M, 700 + i) for i in range(4)]
+    fb = [random_field(m, M, 800 + i) for i in range(4)]
+    bounds, alams = "dddn", [1000.0, 1000.0, 1000.0, 0.0]
+    ra = [torch.from_numpy(x).cuda() for x in fa]
+    rb = [torch.from_numpy(x).cuda() for x in fb]
+    ref = []
+    for rhs in (ra, rb):
+        sol = [torch.zeros_like(r) for r in rhs]
+        dev.helm3d_batch(rhs, sol, bounds, alams)
+        torch.cuda.synchronize()
+        ref.append([x.cpu().numpy() for x in sol])
+    for rep in range(3):
+        sa = [torch.zeros_like(r) for r in ra]
+        sb = [torch.zeros_like(r) for r in rb]
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        s1.wait_stream(torch.cuda.current_stream())
+        s2.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s1):
+            dev.helm3d_batch(ra, sa, bounds, alams)
+        with torch.cuda.stream(s2):
+            dev.helm3d_batch(rb, sb, bounds, alams)
+        torch.cuda.synchronize()
+        for f in range(4):
+            assert np.array_equal(sa[f].cpu().numpy(), ref[0][f]), (rep, "a", f)
+            assert np.array_equal(sb[f].cpu().numpy(), ref[1][f]), (rep, "b", f)

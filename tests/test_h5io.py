@@ -189,3 +189,22 @@ def test_copy_reads_the_same(tmp_path):
     dst = tmp_path / "uvwp_0007.h5"
     shutil.copy(F17, dst)
     np.testing.assert_array_equal(h5io.read_field(str(dst), "u", 17), exact_field(0, 3, 17))
+
+
+def test_short_layout_message_is_an_error(tmp_path):
+    """A data layout message too short for the fields its version and class imply (here: a compact
+    layout of 2 bytes, whose size field would sit at bytes 2-3) is reported as truncated; the parser
+    reads no byte past the message. The object header stays walkable: the bytes the message gives
+    up become a NIL message."""
+    data = bytearray(open(F17, "rb").read())
+    at = data.find(bytes([0x08, 0x00, 0x18, 0x00, 0, 0, 0, 0, 0x03, 0x01]))   # v1 header, layout v3 contiguous
+    assert at > 0
+    data[at + 2:at + 4] = (2).to_bytes(2, "little")        # message size 2
+    data[at + 8:at + 10] = bytes([0x03, 0x00])             # version 3, class 0 (compact)
+    nil = at + 8 + 2                                       # 22 bytes left of the original 32
+    data[nil:nil + 8] = bytes([0x00, 0x00]) + (14).to_bytes(2, "little") + bytes(4)
+    data[nil + 8:nil + 22] = bytes(14)
+    bad = tmp_path / "short_layout.h5"
+    bad.write_bytes(bytes(data))
+    with pytest.raises(_lib.DuctError, match="truncated layout message"):
+        h5io.read_fields(str(bad))
