@@ -46,7 +46,7 @@ struct LpqArgs {
 hipError_t launch_lpq(int NH, const LpqArgs& a, hipStream_t st);
 constexpr int kUDig = 3;    // digits of a quantised quarter solution
 // fp16 hi/lo fragments of an NH x NH block B (row i, column k, column-major with leading dim ld);
-// returns the scale s (B * s has max magnitude in [0.5, 1)).
+// returns the scale s (B * s has max magnitude in [2^11, 2^12); hi = fp16(B s), lo = fp16(B s - hi)).
 float build_lpq_basis(const double* B, int ld, int NH, std::vector<unsigned short>& frag);
 
 // Folded FP64 residual r = f - (S_r u + u S_c^T - xlambda u) on every quarter, u = sum of

@@ -26,6 +26,8 @@
 
 namespace duct {
 
+constexpr int kSplitExp = 12;   // the fp16 hi/lo split works on values scaled to [2^11, 2^12)
+
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 typedef _Float16 h4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -177,24 +179,31 @@ hipError_t launch_unfold_sum(const FieldGroup& g, int NH, const float* const* sr
 // MFMA A operand (A[j][k] = X[k][j]) and B as the B operand (Bop[k][i] = B[i][k]),
 // so D[j][i] = Y[j][i] lands with four consecutive j per lane: stored column by
 // column (column i, rows j) it is the next pass's X without a transpose.  X lives
-// in LDS as two fp16 terms (hi = fp16(x s), lo = fp16((x s - hi) 2048), s a power
-// of two per pass); the products hi*hi go to one FP32 accumulator, hi*lo + lo*hi
-// to another (weight 1/2048).  Lane layout of v_mfma_f32_16x16x32_f16:
+// in LDS as two fp16 terms, hi = fp16(x s) and lo = fp16(x s - hi), with s a power
+// of two per pass that puts max |x s| in [2^11, 2^12): hi then carries 11 bits, lo
+// the next 11 or more, and a lo below fp16's normal range is below 2^-25 of the
+// maximum.  The products hi*hi, hi*lo and lo*hi go to one FP32 accumulator (the
+// bases are split the same way).  Lane layout of v_mfma_f32_16x16x32_f16:
 // A[i][k] lane i + 16 (k/8) element k%8, B[k][j] lane j + 16 (k/8) element k%8,
 // D[i][j] lane j + 16 (i/4) element i%4.
 // Passes: 0 B = P_r^-1, 1 B = P_c^-1 then ./ (lam_r + lam_c - xlambda),
 // 2 B = P_r, 3 B = P_c -> U(y', z').
-template <int NH>
-__global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
+// Work split: NW waves in a 2 x NW/2 grid over the quarter (j x i).  With NW = 4 a wave owns 64 x 64
+// and two workgroups share a CU, so one workgroup's epilogue (VALU, LDS writes, barriers) runs while
+// the other's products keep the matrix cores busy; the B fragments stream one k chunk ahead.
+template <int NH, int NW>
+__global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
 {
+    constexpr int NTH = 64 * NW;
     constexpr int LDX = NH + 8;          // halves per LDS column
-    constexpr int TJ = NH / 32;          // MFMA tiles per wave along j (2 waves along j)
-    constexpr int TI = NH / 64;          // MFMA tiles per wave along i (4 waves along i)
+    constexpr int WI = NW / 2;           // waves along i (2 along j)
+    constexpr int TJ = NH / 32;          // MFMA tiles per wave along j
+    constexpr int TI = NH / (16 * WI);   // MFMA tiles per wave along i
     constexpr int KC = NH / 32;          // k chunks per pass
     constexpr int NT = NH / 16;          // 16-row tiles of a basis
-    constexpr int PER = NH * NH / 2048;  // float4 loads per thread
+    constexpr int PER = NH * NH / (4 * NTH);   // float4 loads per thread
     __shared__ __attribute__((aligned(16))) _Float16 Xs[2][NH * LDX];
-    __shared__ float red[8];
+    __shared__ float red[NW];
     __shared__ float lam_s[2 * NH];      // [lam_e | lam_o] for the divide of pass 1
 
     const FieldGroup& g = a.g;
@@ -214,13 +223,13 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
 
     if (part == 1 && g.mode0 + mode == 0) {   // Im of mode 0 is not solved (int2d3d.f90:43)
         if (a.Ucol) {
-            for (int e = tid; e < kUDig * NH * NH / 16; e += 512) {
+            for (int e = tid; e < kUDig * NH * NH / 16; e += NTH) {
                 ((uint4*)(a.Ucol + qoff * kUDig))[e] = uint4{0u, 0u, 0u, 0u};
                 ((uint4*)(a.Urow + qoff * kUDig))[e] = uint4{0u, 0u, 0u, 0u};
             }
             if (tid == 0) a.gexp[plane * 4 + q] = 0;
         }
-        for (int e = tid; e < NH * NH / 4; e += 512) ((f4v*)Y)[e] = f4v{0.f, 0.f, 0.f, 0.f};
+        for (int e = tid; e < NH * NH / 4; e += NTH) ((f4v*)Y)[e] = f4v{0.f, 0.f, 0.f, 0.f};
         return;
     }
     const double xl64 = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
@@ -233,14 +242,14 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
         __syncthreads();
         float mx = red[0];
 #pragma unroll
-        for (int w = 1; w < 8; ++w) mx = fmaxf(mx, red[w]);
+        for (int w = 1; w < NW; ++w) mx = fmaxf(mx, red[w]);
         return mx;
     };
-    auto scale_of = [](float mx) -> float {   // power of two s with mx * s in [0.5, 1)
+    auto scale_of = [](float mx) -> float {   // power of two s with mx * s in [2^11, 2^12)
         if (!(mx > 0.f) || !(mx < 3.0e38f)) return 1.f;
         int e;
         (void)frexpf(mx, &e);
-        return ldexpf(1.f, -e);
+        return ldexpf(1.f, kSplitExp - e);
     };
     auto put4 = [&](int col, int row, f4v v, float s) {   // 4 consecutive rows of a column
         h4v hi, lo;
@@ -249,94 +258,95 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
             const float x = v[e] * s;
             const _Float16 h = (_Float16)x;
             hi[e] = h;
-            lo[e] = (_Float16)((x - (float)h) * 2048.f);
+            lo[e] = (_Float16)(x - (float)h);
         }
         *(h4v*)&Xs[0][col * LDX + row] = hi;
         *(h4v*)&Xs[1][col * LDX + row] = lo;
     };
 
     // ---- load X (column z', rows y'), scale and split
-    for (int e = tid; e < 2 * NH; e += 512) lam_s[e] = a.lam[e];   // visible after the barriers below
+    for (int e = tid; e < 2 * NH; e += NTH) lam_s[e] = a.lam[e];   // visible after the barriers below
     float sx;
     {
         f4v xv[PER];
         float mx = 0.f;
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            xv[u] = ((const f4v*)X)[tid + 512 * u];
+            xv[u] = ((const f4v*)X)[tid + NTH * u];
 #pragma unroll
             for (int e = 0; e < 4; ++e) mx = fmaxf(mx, fabsf(xv[u][e]));
         }
         sx = scale_of(block_max(mx));
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int idx = (tid + 512 * u) * 4;
+            const int idx = (tid + NTH * u) * 4;
             put4(idx / NH, idx % NH, xv[u], sx);
         }
         __syncthreads();
     }
 
-    const int wj = wave & 1, wi = wave >> 1;   // 2 x 4 waves
-    const int j0 = wj * (NH / 2), i0 = wi * (NH / 4);
-    // the B fragments of a whole pass are loaded at once (2 terms x TI tiles x KC chunks),
-    // the next pass's issued before this pass's epilogue and its barriers
-    h8v bq[2][TI][KC];
-    auto load_b = [&](int pass) {
+    const int wj = wave & 1, wi = wave >> 1;   // 2 x WI waves
+    const int j0 = wj * (NH / 2), i0 = wi * (NH / WI);
+    // B fragments of one k chunk (2 terms x TI tiles), the next chunk's loaded while this one's
+    // products run; the next pass's first chunk is issued before this pass's epilogue
+    h8v bq[2][2][TI];
+    auto load_b = [&](int pass, int kc, h8v (&dst)[2][TI]) {
         const int par = (pass == 0 || pass == 2) ? r : c;
         const h8v* Bf = (const h8v*)(pass < 2 ? a.fwd[par] : a.bwd[par]);
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int y = 0; y < TI; ++y)
-#pragma unroll
-                for (int kc = 0; kc < KC; ++kc)
-                    bq[t][y][kc] = Bf[((size_t)(t * NT + i0 / 16 + y) * KC + kc) * 64 + lane];
+            for (int y = 0; y < TI; ++y) dst[t][y] = Bf[((size_t)(t * NT + i0 / 16 + y) * KC + kc) * 64 + lane];
     };
-    load_b(0);
+    load_b(0, 0, bq[0]);
+    f4v am[TJ][TI];
+    float inv = 1.f, mx = 0.f;
+    // passes 0-2 end in the split into LDS; pass 3's epilogue (the stores) follows the loop, so its
+    // addresses are not loop invariants held through all four passes
     for (int pass = 0; pass < 4; ++pass) {
         const int par = (pass == 0 || pass == 2) ? r : c;
         const float sb = pass < 2 ? a.sfwd[par] : a.sbwd[par];
-        f4v am[TJ][TI], ac[TJ][TI];
 #pragma unroll
         for (int x = 0; x < TJ; ++x)
 #pragma unroll
-            for (int y = 0; y < TI; ++y) am[x][y] = ac[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+            for (int y = 0; y < TI; ++y) am[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc) {
-            h8v ah[TJ], al[TJ];
+            h8v (&bc)[2][TI] = bq[kc & 1];
+            if (kc + 1 < KC) load_b(pass, kc + 1, bq[(kc + 1) & 1]);
+            else if (pass < 3) load_b(pass + 1, 0, bq[(kc + 1) & 1]);
+            // one j tile's A fragments at a time (the registers go to the accumulator); the three
+            // products into am[x][y] are TI launches apart, not back to back
 #pragma unroll
-            for (int t = 0; t < TJ; ++t) {
-                const int j = j0 + t * 16 + l16;
+            for (int x = 0; x < TJ; ++x) {
+                const int j = j0 + x * 16 + l16;
                 const int k = kc * 32 + l4 * 8;
-                ah[t] = *(const h8v*)&Xs[0][j * LDX + k];
-                al[t] = *(const h8v*)&Xs[1][j * LDX + k];
+                const h8v ah = *(const h8v*)&Xs[0][j * LDX + k];
+                const h8v al = *(const h8v*)&Xs[1][j * LDX + k];
+#pragma unroll
+                for (int y = 0; y < TI; ++y)
+                    am[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bc[0][y], am[x][y], 0, 0, 0);
+#pragma unroll
+                for (int y = 0; y < TI; ++y)
+                    am[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bc[1][y], am[x][y], 0, 0, 0);
+#pragma unroll
+                for (int y = 0; y < TI; ++y)
+                    am[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bc[0][y], am[x][y], 0, 0, 0);
             }
-            // the two products into ac are TJ TI launches apart, not back to back
-#pragma unroll
-            for (int y = 0; y < TI; ++y)
-#pragma unroll
-                for (int x = 0; x < TJ; ++x) {
-                    am[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[x], bq[0][y][kc], am[x][y], 0, 0, 0);
-                    ac[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[x], bq[1][y][kc], ac[x][y], 0, 0, 0);
-                }
-#pragma unroll
-            for (int y = 0; y < TI; ++y)
-#pragma unroll
-                for (int x = 0; x < TJ; ++x)
-                    ac[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[x], bq[0][y][kc], ac[x][y], 0, 0, 0);
         }
-        if (pass < 3) load_b(pass + 1);
-        // true values: (am + ac / 2048) inv, inv = 1 / (sx sb) a power of two that is applied
+        // KC is even: the next pass's first chunk sits in bq[0]
+        static_assert(KC % 2 == 0, "the B chunk double buffer assumes an even chunk count");
+        // true values: am inv, inv = 1 / (sx sb) a power of two that is applied
         // together with the next pass's scale; D[j][i]: lane i + 16 (j/4), element j%4
-        const float inv = 1.f / (sx * sb);
-        float mx = 0.f;
+        inv = 1.f / (sx * sb);
+        mx = 0.f;
 #pragma unroll
         for (int x = 0; x < TJ; ++x)
 #pragma unroll
             for (int y = 0; y < TI; ++y)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    float t = am[x][y][e] + ac[x][y][e] * (1.f / 2048.f);
+                    float t = am[x][y][e];
                     if (pass == 1) {   // G = ... ./ (lam_r(j) + lam_c(i) - xlambda), solvediag.f90:61-66, :151-159
                         const int j = j0 + x * 16 + l4 * 4 + e, i = i0 + y * 16 + l16;
                         const bool ok = j < hh[r] && i < hh[c] && !(gauge && j == a.gauge && i == a.gauge);
@@ -347,13 +357,24 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
                     mx = fmaxf(mx, fabsf(t));
                 }
         mx *= inv;
-        if (pass == 3) {
+        if (pass == 3) break;
+        sx = scale_of(block_max(mx));   // the barrier inside: every wave is done reading Xs
+        const float sput = inv * sx;     // a power of two
 #pragma unroll
-            for (int x = 0; x < TJ; ++x)
+        for (int x = 0; x < TJ; ++x)
 #pragma unroll
-                for (int y = 0; y < TI; ++y) am[x][y] *= inv;
-        }
-        if (pass == 3) {   // U(y' = j, z' = i) -> column i, rows j
+            for (int y = 0; y < TI; ++y) {
+                const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
+                put4(i, j, am[x][y], sput);
+            }
+        __syncthreads();
+    }
+    // ---- pass 3's epilogue: U(y' = j, z' = i) -> column i, rows j
+#pragma unroll
+    for (int x = 0; x < TJ; ++x)
+#pragma unroll
+        for (int y = 0; y < TI; ++y) am[x][y] *= inv;
+    {
             if (!a.Ucol) {
 #pragma unroll
                 for (int x = 0; x < TJ; ++x)
@@ -362,16 +383,22 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
                         const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
                         *(f4v*)&Y[(size_t)i * NH + j] = am[x][y];
                     }
-                break;
+                return;
             }
             // digit form: Z = rint(U 2^(24-g)), |Z| < 0.49 2^24, as signed base-256 digits
-            const float gmx = block_max(mx);   // the barrier inside: every wave is done reading Xs
+            const float gmx = block_max(mx);
             int ge = 0;
             if (gmx > 0.f && gmx < 3.0e38f) (void)frexpf(gmx / 0.49f, &ge);
             const float sc = ldexpf(1.f, 24 - ge);
             const float isc = ldexpf(1.f, ge - 24);
-            unsigned char* Xb = (unsigned char*)&Xs[0][0];   // [d][y'][z'], kUDig NH^2 bytes
             unsigned char* uc = a.Ucol + qoff * kUDig;
+            unsigned char* ur = a.Urow + qoff * kUDig;
+            // the 4 x 4 byte transpose within a lane quad (lanes l16 = 4 a .. 4 a + 3 hold columns
+            // i .. i + 3 of rows j .. j + 3): quad_perm [1,0,3,2] and a byte select, then [2,3,0,1]
+            // and a half-word select leave row j + (l16 % 4), columns i .. i + 3 in each lane
+            const int qv = l16 & 3;
+            const unsigned selA = (qv & 1) ? 0x03070105u : 0x06020400u;
+            const unsigned selB = (qv & 2) ? 0x03020706u : 0x05040100u;
 #pragma unroll
             for (int x = 0; x < TJ; ++x)
 #pragma unroll
@@ -389,31 +416,20 @@ __global__ __launch_bounds__(512, 2) void k_lpq(LpqArgs a)
                         const int c1 = (Z - c2) >> 8;
                         const int cd[kUDig] = {c1, c2, c3};
 #pragma unroll
-                        for (int d = 0; d < kUDig; ++d) {
-                            w[d] |= (unsigned)(cd[d] & 255) << (8 * e);
-                            Xb[((size_t)d * NH + j + e) * NH + i] = (unsigned char)(cd[d] & 255);
-                        }
+                        for (int d = 0; d < kUDig; ++d) w[d] |= (unsigned)(cd[d] & 255) << (8 * e);
                     }
 #pragma unroll
-                    for (int d = 0; d < kUDig; ++d) *(unsigned*)&uc[((size_t)d * NH + i) * NH + j] = w[d];
+                    for (int d = 0; d < kUDig; ++d) {
+                        *(unsigned*)&uc[((size_t)d * NH + i) * NH + j] = w[d];
+                        const unsigned wx = (unsigned)__builtin_amdgcn_update_dpp(0, (int)w[d], 0xB1, 0xF, 0xF, false);
+                        const unsigned t1 = __builtin_amdgcn_perm(wx, w[d], selA);
+                        const unsigned tx = (unsigned)__builtin_amdgcn_update_dpp(0, (int)t1, 0x4E, 0xF, 0xF, false);
+                        const unsigned t2 = __builtin_amdgcn_perm(tx, t1, selB);
+                        *(unsigned*)&ur[((size_t)d * NH + j + qv) * NH + (i - qv)] = t2;
+                    }
                     *(f4v*)&Y[(size_t)i * NH + j] = uq;
                 }
-            __syncthreads();
-            unsigned char* ur = a.Urow + qoff * kUDig;
-            for (int e = tid; e < kUDig * NH * NH / 16; e += 512) ((uint4*)ur)[e] = ((const uint4*)Xb)[e];
             if (tid == 0) a.gexp[plane * 4 + q] = ge;
-            break;
-        }
-        sx = scale_of(block_max(mx));   // the barrier inside: every wave is done reading Xs
-        const float sput = inv * sx;     // a power of two
-#pragma unroll
-        for (int x = 0; x < TJ; ++x)
-#pragma unroll
-            for (int y = 0; y < TI; ++y) {
-                const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
-                put4(i, j, am[x][y], sput);
-            }
-        __syncthreads();
     }
 }
 
@@ -421,10 +437,10 @@ hipError_t launch_lpq(int NH, const LpqArgs& a, hipStream_t st)
 {
     const long planes = (long)a.g.nf * a.g.nmodes * 2;
     if (planes <= 0) return hipSuccess;
-    const dim3 grid((unsigned)(planes * 4)), block(512);
+    const dim3 grid((unsigned)(planes * 4));
     switch (NH) {
-        case 64: hipLaunchKernelGGL((k_lpq<64>), grid, block, 0, st, a); break;
-        case 128: hipLaunchKernelGGL((k_lpq<128>), grid, block, 0, st, a); break;
+        case 64: hipLaunchKernelGGL((k_lpq<64, 4>), grid, dim3(256), 0, st, a); break;
+        case 128: hipLaunchKernelGGL((k_lpq<128, 4>), grid, dim3(256), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -438,7 +454,7 @@ float build_lpq_basis(const double* B, int ld, int NH, std::vector<unsigned shor
         for (int i = 0; i < NH; ++i) mx = std::fmax(mx, std::fabs(B[i + (size_t)k * ld]));
     int ex = 0;
     if (mx > 0.0) (void)std::frexp(mx, &ex);
-    const float s = std::ldexp(1.f, -ex);
+    const float s = std::ldexp(1.f, kSplitExp - ex);
     const int NT = NH / 16, KC = NH / 32;
     frag.assign((size_t)2 * NT * KC * 64 * 8, 0);
     for (int it = 0; it < NT; ++it)
@@ -448,7 +464,7 @@ float build_lpq_basis(const double* B, int ld, int NH, std::vector<unsigned shor
                     const int i = it * 16 + (lane & 15), k = kc * 32 + (lane >> 4) * 8 + e;
                     const float x = (float)B[i + (size_t)k * ld] * s;
                     const _Float16 h = (_Float16)x;
-                    const _Float16 l = (_Float16)((x - (float)h) * 2048.f);
+                    const _Float16 l = (_Float16)(x - (float)h);
                     unsigned short hb, lb;
                     std::memcpy(&hb, &h, 2);
                     std::memcpy(&lb, &l, 2);
