@@ -26,6 +26,7 @@ SIGNATURES = {
     "duct_setup_": (_i, [_pc, _pi, _vp, _vp, _vp, _vp, _vp, _pi, _pd, _sz]),
     "duct_helm3d_dev_": (_i, [_psz, _psz, _pi, _pi, _pi, _pd, _pd, _pc, _sz]),
     "duct_helm3d_batch_dev_": (_i, [_pi, _psz, _psz, _pc, _pd, _pi, _pi, _pi, _pd, _sz]),
+    "duct_helm3d_batch_noalloc_dev_": (_i, [_pi, _psz, _psz, _pc, _pd, _pi, _pi, _pi, _pd, _sz]),
     "duct_helm3d_": (_i, [_vp, _vp, _pi, _pi, _pi, _pd, _pd, _pc, _sz]),
     "duct_int2d3d_": (_i, [_vp, _vp, _pc, _pd, _pi, _sz]),
     "duct_solvediag_": (_i, [_vp, _pc, _pd, _sz]),

@@ -30,6 +30,15 @@
 
 using namespace duct;
 
+// Scratch of the allocation-free entry duct_helm3d_batch_noalloc_dev_: the two planar buffers of the
+// projection passes and the Neumann edge sums of one boundary group, in a fixed device array.  Every
+// GPU the library runs on holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel or a copy (that is a host-side stand-in, not a device address): it uses
+// the per-device address that hipGetSymbolAddress returns (Lib::scratch_ptr).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
 
 struct DevBasis {
@@ -138,6 +147,8 @@ struct Lib {
     d2v* prods = nullptr;                 // the six product fields of the advection term
     size_t prods_bytes = 0;
     hipEvent_t ev_switch = nullptr;       // orders a newly set stream after the previous one
+    bool noalloc = false;                 // inside duct_helm3d_batch_noalloc_dev_: nothing may allocate
+    void* scratch_ptr[kNoAllocDevices] = {};  // per device: the address of the static scratch array
     float* mfold = nullptr;               // folded FP32 arrays of the folded mixed solve
     size_t mfold_bytes = 0;
     unsigned char* mdig = nullptr;        // its digit-form iterate (Ucol, Urow) and exponents
@@ -215,6 +226,19 @@ int fail(int code, const char* fmt, ...)
                         __FILE__, __LINE__);                                              \
     } while (0)
 
+// The device address of the static scratch array on `device` (current), looked up once per device.
+int noalloc_scratch_setup(int device)
+{
+    if (device < 0 || device >= kNoAllocDevices) return OK;   // the entry then reports it
+    if (L.scratch_ptr[device]) return OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    L.scratch_ptr[device] = ptr;
+    return OK;
+}
+
 int ensure_init()
 {
     if (L.init) return OK;
@@ -229,6 +253,8 @@ template <class T>
 int ensure_buf(T*& p, size_t& cap, size_t bytes)
 {
     if (bytes <= cap && p) return OK;
+    if (L.noalloc)
+        return fail(ERR_ARG, "the allocation-free entry needs %zu bytes of a buffer that holds %zu", bytes, cap);
     if (p) {
         HIPCHK(hipStreamSynchronize(L.stream));
         HIPCHK(hipFree(p));
@@ -247,6 +273,9 @@ int ensure_buf(T*& p, size_t& cap, size_t bytes)
 int ensure_planar(size_t bytes)
 {
     if (bytes <= L.plan_bytes && L.plan[0]) return OK;
+    if (L.noalloc)
+        return fail(ERR_ARG, "the allocation-free entry needs %zu planar bytes, its scratch holds %zu", bytes,
+                    L.plan_bytes);
     size_t c0 = L.plan_bytes, c1 = L.plan_bytes;
     int st = ensure_buf(L.plan[0], c0, bytes);
     if (st) return st;
@@ -284,7 +313,7 @@ struct Timed {
     hipEvent_t a = nullptr, b = nullptr;
     explicit Timed(int k) : kind(k)
     {
-        if (!L.prof) return;
+        if (!L.prof || L.noalloc) return;   // events come from a pool that may create them
         a = take_event();
         b = take_event();
         if (a) (void)hipEventRecord(a, L.stream);
@@ -1116,7 +1145,7 @@ int duct_init_(const int* device)
     HIPCHK(hipSetDevice(*device));
     L.device = *device;
     L.init = true;
-    return OK;
+    return noalloc_scratch_setup(*device);
 }
 
 int duct_shutdown_(void)
@@ -1503,6 +1532,75 @@ int duct_helm3d_batch_dev_(const int* nfields, const duct_devptr_t* rhs, const d
     int st = ensure_init();
     if (st) return st;
     return solve_fields(*nfields, rhs, sol, bounds, alambda, *m, *nmodes, *ib, *alp, BND_DIR_ZERO);
+}
+
+// duct_helm3d_batch_dev_ with the planar intermediates and the Neumann edge sums in the static scratch
+// array: no allocation of any kind.  The unfolded FP64 passes (the reference's products), the two
+// boundary groups one after the other on the caller's stream, no per-launch events.  Every kernel
+// writes the scratch it reads, so nothing in it needs clearing; calls that overlap on one GPU race.
+int duct_helm3d_batch_noalloc_dev_(const int* nfields, const duct_devptr_t* rhs, const duct_devptr_t* sol,
+                                   const char* bounds, const double* alambda, const int* m, const int* nmodes,
+                                   const int* ib, const double* alp, size_t bounds_len)
+{
+    if (!nfields || !rhs || !sol || !bounds || !alambda || !m || !nmodes || !ib || !alp)
+        return fail(ERR_ARG, "null argument to duct_helm3d_batch_noalloc_dev_");
+    if (bounds_len && (size_t)*nfields > bounds_len)
+        return fail(ERR_ARG, "bounds has %zu characters for %d fields", bounds_len, *nfields);
+    int st = ensure_init();
+    if (st) return st;
+    if (L.device < 0 || L.device >= kNoAllocDevices)
+        return fail(ERR_UNSUPPORTED, "device %d: the allocation-free entry serves devices 0..%d", L.device,
+                    kNoAllocDevices - 1);
+    if ((st = noalloc_scratch_setup(L.device))) return st;
+    if (*nfields < 1 || *nfields > kMaxFields) return fail(ERR_ARG, "nfields=%d outside [1, %d]", *nfields, kMaxFields);
+    if ((st = check_common(*m, *nmodes))) return st;
+    // the largest boundary group sets the scratch: 2 planar buffers + the Neumann edge sums
+    size_t need_plan = 0, need_bnd = 0;
+    for (int bidx = 0; bidx < 2; ++bidx) {
+        FieldGroup g{};
+        g.nmodes = *nmodes; g.mode0 = *ib; g.m = *m; g.n = *m - 2; g.alp = *alp;
+        for (int f = 0; f < *nfields; ++f) {
+            const int bi = bound_index(bounds[f]);
+            if (bi < 0) return fail(ERR_ARG, "bound '%c' of field %d is neither 'd' nor 'n'", bounds[f], f);
+            if (bi == bidx) ++g.nf;
+        }
+        if (!g.nf) continue;
+        const DevBasis& B = L.basis[bidx];
+        if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
+        const size_t planes = (size_t)g.nf * g.nmodes * 2;
+        need_plan = std::max(need_plan, planes * B.NP * B.NP * sizeof(double));
+        if (bidx == 1) need_bnd = boundary_scratch_bytes(g);
+    }
+    const size_t plan_al = (need_plan + 255) & ~(size_t)255;
+    if (2 * plan_al + need_bnd > kNoAllocScratchBytes)
+        return fail(ERR_ARG, "the allocation-free entry holds %zu bytes of scratch; this batch (%d fields x %d "
+                    "modes, m=%d) needs %zu", kNoAllocScratchBytes, *nfields, *nmodes, *m, 2 * plan_al + need_bnd);
+    unsigned char* base = (unsigned char*)L.scratch_ptr[L.device];
+    // the library's own buffers step aside for the call
+    double* plan_keep[2] = {L.plan[0], L.plan[1]};
+    const size_t plan_bytes_keep = L.plan_bytes;
+    d2v* bscr_keep = L.bscr;
+    const size_t bscr_bytes_keep = L.bscr_bytes;
+    const int fold_keep = L.fold;
+    L.plan[0] = (double*)base;
+    L.plan[1] = (double*)(base + plan_al);
+    L.plan_bytes = plan_al;
+    L.bscr = need_bnd ? (d2v*)(base + 2 * plan_al) : nullptr;
+    L.bscr_bytes = need_bnd;
+    L.fold = 0;
+    L.noalloc = true;
+    const int overlap_keep = L.overlap;
+    L.overlap = 0;
+    st = solve_fields(*nfields, rhs, sol, bounds, alambda, *m, *nmodes, *ib, *alp, BND_DIR_ZERO);
+    L.overlap = overlap_keep;
+    L.noalloc = false;
+    L.fold = fold_keep;
+    L.plan[0] = plan_keep[0];
+    L.plan[1] = plan_keep[1];
+    L.plan_bytes = plan_bytes_keep;
+    L.bscr = bscr_keep;
+    L.bscr_bytes = bscr_bytes_keep;
+    return st;
 }
 
 int duct_helm3d_(double* rhs, double* sol, const int* m, const int* nmodes, const int* ib, const double* alp,

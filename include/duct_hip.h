@@ -115,6 +115,24 @@ int duct_int2d3d_(double *sol, const double *rhs, const char *bound, const doubl
  * callers such as precomp_mass_flow (src/operators.F:2276).               */
 int duct_solvediag_(double *U, const char *bound, const double *xlambda, size_t bound_len);
 
+/* duct_helm3d_batch_dev_ for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The scratch (two planar
+ * buffers of 2 nfields nmodes NP^2 doubles for the larger boundary group, NP
+ * the interior padded to 32/64/128/256/512, plus the Neumann edge sums) lives
+ * in one static device array of 64 MiB per GPU, held until the process ends;
+ * a batch that needs more is rejected with DUCT_ERR_ARG (m = 129: 0.5 MiB per
+ * field-mode, about 120 field-modes; m = 257: 2 MiB, about 30).  Same results
+ * as duct_helm3d_batch_dev_
+ * with the unfolded FP64 passes (duct_set_fold_(0)); the two boundary groups
+ * run one after the other on the library's stream.  Calls that overlap on one
+ * GPU share the array and race.  Nothing is allocated: duct_init_ and
+ * duct_setup_ must have run.                                               */
+int duct_helm3d_batch_noalloc_dev_(const int *nfields, const duct_devptr_t *rhs,
+                                   const duct_devptr_t *sol, const char *bounds,
+                                   const double *alambda, const int *m, const int *nmodes,
+                                   const int *ib, const double *alp, size_t bounds_len);
+
 /* Mixed-precision refined variant of duct_helm3d_batch_dev_ (same fields,
  * boundary treatment and results to within the refinement tolerance).  The
  * reference solves in FP64 only (solvediag.f90:51-80, :140-219); here the four
