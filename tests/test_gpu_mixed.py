@@ -152,19 +152,20 @@ def test_mixed_full_size_config2(dev, oracle_mod):
         oracle_mod.use_blas(False)
 
 
-@pytest.mark.parametrize("scheme", ["fold16", "fold16q"])
-def test_fold16_full_size_config2(dev, oracle_mod, scheme):
-    """257^2 x 256 modes x 4 fields, the folded fused solve with one refinement (scheme 5: FP64-MFMA
-    residual; scheme 6: the residual from exact int8 digit products): <= 1e-10 on sampled modes, the
-    low Poisson modes included."""
+@pytest.mark.parametrize("scheme,niter", [("fold16", 1), ("fold16q", 1), ("f32", 1), ("f32", 2)])
+def test_fold16_full_size_config2(dev, oracle_mod, scheme, niter):
+    """257^2 x 256 modes x 4 fields: the folded fused solve with one refinement (scheme 5: FP64-MFMA
+    residual; scheme 6: the residual from exact int8 digit products), and the FP32-MFMA eigenbasis
+    products (scheme 1) with one and two FP64 refinements: <= 1e-10 on sampled modes, the low
+    Poisson modes included."""
     from duct_amd import workloads
     cfg = workloads.SOLVE_257
     ensure(dev, cfg.m)
     ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(cfg.m))
     fields = [workloads.rhs_field(cfg, f) for f in range(4)]
     alams = [workloads.alambda_of(cfg, f) for f in range(4)]
-    got, st = run_mixed(dev, fields, workloads.BOUNDS, alams, scheme0=scheme, scheme1=scheme, niter=1)
-    print("%s residual history:" % scheme, [" ".join("%.1e" % (r / fm) for r, fm in st[:, f]) for f in range(4)],
+    got, st = run_mixed(dev, fields, workloads.BOUNDS, alams, scheme0=scheme, scheme1=scheme, niter=niter)
+    print("%s x%d residual history:" % (scheme, niter), [" ".join("%.1e" % (r / fm) for r, fm in st[:, f]) for f in range(4)],
           flush=True)
     oracle_mod.use_blas(True)
     try:
@@ -172,7 +173,7 @@ def test_fold16_full_size_config2(dev, oracle_mod, scheme):
             for k0 in (0, 1, 2, 7, 8, 100, 255):
                 ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
                 e = relerr(got[f][k0:k0 + 1], ref)
-                print("%s 257^2x256 field %s mode %d rel.err %.2e" % (scheme, workloads.FIELDS[f], k0, e),
+                print("%s x%d 257^2x256 field %s mode %d rel.err %.2e" % (scheme, niter, workloads.FIELDS[f], k0, e),
                       flush=True)
                 assert e <= TOL
     finally:
