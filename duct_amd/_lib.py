@@ -39,8 +39,8 @@ SIGNATURES = {
     "duct_transyz_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _sz]),
     "duct_fourx_dev_": (_i, [_psz, _psz, _pi, _pi, _pi, _pi, _pi]),
     "duct_rk3_substep_dev_": (_i, [_psz] * 7 + [_pi, _pi, _pi, _pd, _pd, _pd, _pd, _pi, _pd]),
-    "duct_setup_temp_": (_i, [_pc, _pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
-    "duct_helm3d_temp_dev_": (_i, [_psz, _psz, _psz, _psz, _pi, _pi, _pi, _pd, _pd]),
+    "duct_setup_temp_": (_i, [_pc, _pi, _pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "duct_helm3d_temp_dev_": (_i, [_psz, _psz, _psz, _psz, _pi, _pi, _pi, _pi, _pd, _pd]),
     "duct_setup_shen_": (_i, [_pc, _pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz]),
     "duct_helm3d_shen_dev_": (_i, [_psz, _psz, _pi, _pi, _pi, _pd, _pd, _pc, _sz]),
     "duct_derivyz_dev_": (_i, [_psz, _psz, _pi, _pi, _pc, _pi, _pd, _pc, _pd, _sz, _sz]),

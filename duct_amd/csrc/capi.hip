@@ -85,7 +85,7 @@ struct DevBasis {
 // Mixed-boundary (temperature) bases: separate y and z eigenbases (pseudomat_temp.f90)
 struct TempDev {
     bool ready = false;
-    int m = 0, NP = 0;
+    int my = 0, mz = 0, NP = 0;
     int b[4] = {0, 0, 0, 0};                 // y=-1, y=+1, z=-1, z=+1: 0 = 'd', 1 = 'n'
     double* Ayi = nullptr;                   // PTYM1, fragment layout (pass 1)
     double* Azi = nullptr;                   // PTZM1 (pass 2)
@@ -93,7 +93,7 @@ struct TempDev {
     double* Az = nullptr;                    // PTZ (pass 4)
     double* ly = nullptr;                    // ETY, NP (zero padded)
     double* lz = nullptr;                    // ETZ
-    double* vec = nullptr;                   // the 8 m wall vectors of TempArgs
+    double* vec = nullptr;                   // the 4 my + 4 mz wall vectors of TempArgs
 };
 
 // Shen-Legendre Galerkin matrices of one boundary type (preshen.f90)
@@ -2147,13 +2147,14 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
 }
 
 // The GPU-upload tails of pseudomaty_temp / pseudomatz_temp (pseudomat_temp.f90:2-321): the
-// caller's y and z bases (interior n x n, column-major), eigenvalues and scaled derivative
-// matrices (m x m); bounds = the four wall types y=-1, y=+1, z=-1, z=+1.
-int duct_setup_temp_(const char* bounds, const int* m_, const double* PY, const double* PYinv, const double* ETY,
-                     const double* DY, const double* PZ, const double* PZinv, const double* ETZ, const double* DZ,
-                     size_t bounds_len)
+// caller's y basis (interior ny x ny, ny = my - 2) and z basis (nz x nz), column-major, their
+// eigenvalues and the scaled derivative matrices (my x my, mz x mz); bounds = the four wall
+// types y=-1, y=+1, z=-1, z=+1.  my and mz may differ (helm3d_temp.f90:35-36).
+int duct_setup_temp_(const char* bounds, const int* my_, const int* mz_, const double* PY, const double* PYinv,
+                     const double* ETY, const double* DY, const double* PZ, const double* PZinv, const double* ETZ,
+                     const double* DZ, size_t bounds_len)
 {
-    if (!bounds || !m_ || !PY || !PYinv || !ETY || !DY || !PZ || !PZinv || !ETZ || !DZ)
+    if (!bounds || !my_ || !mz_ || !PY || !PYinv || !ETY || !DY || !PZ || !PZinv || !ETZ || !DZ)
         return fail(ERR_ARG, "null argument to duct_setup_temp_");
     if (bounds_len < 4) return fail(ERR_ARG, "bounds needs 4 characters (y=-1, y=+1, z=-1, z=+1)");
     int b[4];
@@ -2161,9 +2162,10 @@ int duct_setup_temp_(const char* bounds, const int* m_, const double* PY, const 
         if (bounds[w] != 'd' && bounds[w] != 'n') return fail(ERR_ARG, "wall %d: bound '%c'", w + 1, bounds[w]);
         b[w] = bounds[w] == 'n';
     }
-    const int m = *m_;
-    if (m < kMinM || m > kMaxM) return fail(ERR_UNSUPPORTED, "m=%d outside [%d, %d]", m, kMinM, kMaxM);
-    const int n = m - 2, NP = padded_n(n);
+    const int my = *my_, mz = *mz_;
+    for (int mm : {my, mz})
+        if (mm < kMinM || mm > kMaxM) return fail(ERR_UNSUPPORTED, "m=%d outside [%d, %d]", mm, kMinM, kMaxM);
+    const int ny = my - 2, nz = mz - 2, NP = padded_n(std::max(ny, nz));
     int st = ensure_init();
     if (st) return st;
     TempDev& T = L.temp;
@@ -2174,36 +2176,42 @@ int duct_setup_temp_(const char* bounds, const int* m_, const double* PY, const 
         HIPCHK(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice));
         return OK;
     };
-    std::vector<double> frag((size_t)NP * NP), lp(NP, 0.0), vec((size_t)8 * m, 0.0);
-    build_frag(PYinv, n, n, NP, frag.data());
+    // both bases padded with zeros to the common NP: the passes of one direction then read zero
+    // K columns beyond that direction's size, and the padded plane rows and columns stay zero
+    std::vector<double> frag((size_t)NP * NP), lp(NP, 0.0), vec((size_t)4 * (my + mz), 0.0);
+    build_frag(PYinv, ny, ny, NP, frag.data());
     if ((st = up(T.Ayi, frag.data(), frag.size()))) return st;
-    build_frag(PZinv, n, n, NP, frag.data());
+    build_frag(PZinv, nz, nz, NP, frag.data());
     if ((st = up(T.Azi, frag.data(), frag.size()))) return st;
-    build_frag(PY, n, n, NP, frag.data());
+    build_frag(PY, ny, ny, NP, frag.data());
     if ((st = up(T.Ay, frag.data(), frag.size()))) return st;
-    build_frag(PZ, n, n, NP, frag.data());
+    build_frag(PZ, nz, nz, NP, frag.data());
     if ((st = up(T.Az, frag.data(), frag.size()))) return st;
-    for (int i = 0; i < n; ++i) lp[i] = ETY[i];
+    for (int i = 0; i < ny; ++i) lp[i] = ETY[i];
     if ((st = up(T.ly, lp.data(), lp.size()))) return st;
-    for (int i = 0; i < n; ++i) lp[i] = ETZ[i];
+    std::fill(lp.begin(), lp.end(), 0.0);
+    for (int i = 0; i < nz; ++i) lp[i] = ETZ[i];
     if ((st = up(T.lz, lp.data(), lp.size()))) return st;
     for (int d = 0; d < 2; ++d) {        // D2 = D D columns 1 and m (pseudomat_temp.f90:98-104), rows 1 and m of D
         const double* D = d ? DZ : DY;
+        const int m = d ? mz : my;
+        double* v = vec.data() + (d ? (size_t)4 * my : 0);
         for (int i = 0; i < m; ++i) {
             double s0 = 0.0, s1 = 0.0;
             for (int k = 0; k < m; ++k) {
                 s0 += D[i + (size_t)k * m] * D[k];
                 s1 += D[i + (size_t)k * m] * D[k + (size_t)(m - 1) * m];
             }
-            vec[(size_t)(2 * d) * m + i] = s0;
-            vec[(size_t)(2 * d + 1) * m + i] = s1;
-            vec[(size_t)(4 + 2 * d) * m + i] = D[(size_t)i * m];                 // D(1, i)
-            vec[(size_t)(5 + 2 * d) * m + i] = D[(m - 1) + (size_t)i * m];       // D(m, i)
+            v[i] = s0;
+            v[(size_t)m + i] = s1;
+            v[(size_t)2 * m + i] = D[(size_t)i * m];                   // D(1, i)
+            v[(size_t)3 * m + i] = D[(m - 1) + (size_t)i * m];         // D(m, i)
         }
     }
     if ((st = up(T.vec, vec.data(), vec.size()))) return st;
     for (int w = 0; w < 4; ++w) T.b[w] = b[w];
-    T.m = m;
+    T.my = my;
+    T.mz = mz;
     T.NP = NP;
     T.ready = true;
     return OK;
@@ -2211,23 +2219,25 @@ int duct_setup_temp_(const char* bounds, const int* m_, const double* PY, const 
 
 // helm3d_temp (helm3d_temp.f90:2-44, int2d3d_temp.f90, solvediag_temp.f90) on a device field:
 // sol = the mixed-boundary solution of (D_yy + D_zz - xlambda) u = rhs, xlambda = alambda + (alp k)^2,
-// with the wall data bcy [nmodes][2][m] (over z) and bcz [nmodes][2][m] (over y); rhs is not modified.
+// fields (my, mz, nmodes), the wall data bcy [nmodes][2][mz] (over z) and bcz [nmodes][2][my]
+// (over y); rhs is not modified.
 int duct_helm3d_temp_dev_(const duct_devptr_t* rhs, const duct_devptr_t* sol, const duct_devptr_t* bcy,
-                          const duct_devptr_t* bcz, const int* m, const int* nmodes, const int* ib, const double* alp,
-                          const double* alambda)
+                          const duct_devptr_t* bcz, const int* my, const int* mz, const int* nmodes, const int* ib,
+                          const double* alp, const double* alambda)
 {
-    if (!rhs || !sol || !bcy || !bcz || !m || !nmodes || !ib || !alp || !alambda)
+    if (!rhs || !sol || !bcy || !bcz || !my || !mz || !nmodes || !ib || !alp || !alambda)
         return fail(ERR_ARG, "null argument to duct_helm3d_temp_dev_");
     if (!*rhs || !*sol || !*bcy || !*bcz) return fail(ERR_ARG, "null device pointer");
     if (*rhs == *sol) return fail(ERR_ARG, "rhs and sol must not alias (helm3d_temp keeps the rhs)");
     int st = ensure_init();
     if (st) return st;
-    if ((st = check_common(*m, *nmodes))) return st;
+    if ((st = check_common(*my, *nmodes)) || (st = check_common(*mz, *nmodes))) return st;
     if (*ib < 0) return fail(ERR_ARG, "ib=%d < 0", *ib);
     TempDev& T = L.temp;
-    if (!T.ready || T.m != *m) return fail(ERR_NOT_SETUP, "duct_setup_temp_ not called for m=%d", *m);
-    const int NP = T.NP, mm = *m, M = *nmodes;
-    const size_t field = (size_t)M * mm * mm;
+    if (!T.ready || T.my != *my || T.mz != *mz)
+        return fail(ERR_NOT_SETUP, "duct_setup_temp_ not called for my=%d, mz=%d", *my, *mz);
+    const int NP = T.NP, ny = *my - 2, nz = *mz - 2, M = *nmodes;
+    const size_t field = (size_t)M * *my * *mz;
     if ((st = ensure_buf(L.tfield, L.tfield_bytes, field * sizeof(d2v)))) return st;
     const long planes = (long)M * 2;
     if ((st = ensure_planar((size_t)planes * NP * NP * sizeof(double)))) return st;
@@ -2237,14 +2247,14 @@ int duct_helm3d_temp_dev_(const duct_devptr_t* rhs, const duct_devptr_t* sol, co
     ta.bcy = (const double*)*bcy;
     ta.bcz = (const double*)*bcz;
     ta.vec = T.vec;
-    ta.m = mm; ta.nmodes = M; ta.mode0 = *ib;
+    ta.my = *my; ta.mz = *mz; ta.nmodes = M; ta.mode0 = *ib;
     ta.by1 = T.b[0]; ta.by2 = T.b[1]; ta.bz1 = T.b[2]; ta.bz2 = T.b[3];
     {
         Timed t(PK_BCCORR);
         HIPCHK(launch_temp_rhs(ta, L.stream));
     }
     FieldGroup g{};
-    g.nf = 1; g.nmodes = M; g.mode0 = *ib; g.m = mm; g.n = mm - 2; g.alp = *alp;
+    g.nf = 1; g.nmodes = M; g.mode0 = *ib; g.m = *my; g.n = ny; g.alp = *alp;
     g.in[0] = L.tfield;
     g.out[0] = (d2v*)*sol;
     g.alambda[0] = *alambda;
@@ -2253,24 +2263,30 @@ int duct_helm3d_temp_dev_(const duct_devptr_t* rhs, const duct_devptr_t* sol, co
     a.lam = T.ly;
     a.lam2 = T.lz;
     a.gauge = -1;
-    {   // pass 1: (PTYM1 F)^T
+    a.ldf = *my;
+    a.fplane = (long)*my * *mz;
+    {   // pass 1: (PTYM1 F)^T, K = y, columns z
         Timed t(PK_C2P);
         a.Afrag = T.Ayi; a.X = nullptr; a.Y = L.plan[0]; a.ncols = (long)M * NP;
+        a.kn = ny; a.cn = nz;
         HIPCHK(launch_proj(NP, PROJ_C2P, a, L.stream));
     }
-    {   // pass 2: PTYM1 F PTZM1^T ./ (ETY(i) + ETZ(j) - xlambda) (solvediag_temp.f90:84-96)
+    {   // pass 2: PTYM1 F PTZM1^T ./ (ETY(i) + ETZ(j) - xlambda) (solvediag_temp.f90:84-96), K = z, columns y-eigen
         Timed t(PK_P2P_DIV);
         a.Afrag = T.Azi; a.X = L.plan[0]; a.Y = L.plan[1]; a.ncols = planes * NP;
+        a.kn = nz; a.cn = ny;
         HIPCHK(launch_proj(NP, PROJ_P2P_DIV, a, L.stream));
     }
-    {   // pass 3: (PTY G)^T
+    {   // pass 3: (PTY G)^T, K = y-eigen, columns z-eigen
         Timed t(PK_P2P);
         a.Afrag = T.Ay; a.X = L.plan[1]; a.Y = L.plan[0]; a.ncols = planes * NP;
+        a.kn = ny; a.cn = nz;
         HIPCHK(launch_proj(NP, PROJ_P2P, a, L.stream));
     }
-    {   // pass 4: PTY G PTZ^T into the interior of sol
+    {   // pass 4: PTY G PTZ^T into the interior of sol, K = z-eigen, columns y
         Timed t(PK_P2C);
         a.Afrag = T.Az; a.X = L.plan[0]; a.Y = nullptr; a.ncols = (long)M * NP;
+        a.kn = nz; a.cn = ny;
         HIPCHK(launch_proj(NP, PROJ_P2C, a, L.stream));
     }
     {   // edge_value_y / edge_value_z, corners from the rhs

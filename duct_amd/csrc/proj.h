@@ -38,6 +38,11 @@ struct ProjArgs {
     long ncols;           // planar: planes*NP columns; complex: nf*nmodes*NP complex columns
     int gauge;            // 0-based eigen-index of the Neumann null mode, -1 if none
     FieldGroup g;
+    // Rectangular planes (separate y and z bases of sizes my, mz): the pass's valid K (= output
+    // row) count kn, its valid column count cn, the complex fields' y extent ldf and plane size
+    // fplane (my mz).  All 0: the square plane, kn = cn = g.n, ldf = g.m.
+    int kn, cn, ldf;
+    long fplane;
 };
 
 hipError_t launch_proj(int NP, int mode, const ProjArgs& a, hipStream_t st);

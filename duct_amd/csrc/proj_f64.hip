@@ -65,6 +65,9 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
     const int rowblk = blockIdx.y;
     const FieldGroup& g = a.g;
     const int n = g.n, m = g.m;
+    // valid extents: K and output rows kn, columns cn; complex fields (ldf, ., .) of fpl points
+    const int kn = a.kn ? a.kn : n, cn = a.cn ? a.cn : n, ldf = a.ldf ? a.ldf : m;
+    const size_t fpl = a.ldf ? (size_t)a.fplane : (size_t)m * m;
 
     // ---------------- B-tile loader geometry ----------------
     // planar source (P2P, P2C): each thread loads DPT consecutive k of one column
@@ -78,7 +81,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
     // A load under a branch makes the compiler wait for every outstanding load, the next
     // stage's HBM reads included, before the first instruction that follows the branch.
     const double* srcp = a.X;              // planar column base (k = 0)
-    const d2v* srcc = a.g.in[0] + 1 + a.g.m;   // complex column base (k = 0 -> y = 1)
+    const d2v* srcc = a.g.in[0] + 1 + ldf;     // complex column base (k = 0 -> y = 1)
     int v_dst = 0, v_dst_im = 0, kk0 = 0;
     bool src_ok = false, zero_im = false;
 
@@ -99,7 +102,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
             const long fm = plane >> 1;
             const int f = (int)(fm / g.nmodes), mode = (int)(fm % g.nmodes);
             if (src_ok) ld_xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
-            if (src_ok && ld_j < n) ld_lamj = (a.lam2 ? a.lam2 : a.lam)[ld_j];
+            if (src_ok && ld_j < cn) ld_lamj = (a.lam2 ? a.lam2 : a.lam)[ld_j];
             ld_gauge = a.gauge >= 0 && fabs(ld_xlam) <= 1.0e-10 && ld_j == a.gauge;
         }
     } else if constexpr (COUT) {
@@ -126,9 +129,9 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
             const long fm = q / NP;
             const int j = (int)(q % NP);
             const int f = (int)(fm / g.nmodes), mode = (int)(fm % g.nmodes);
-            src_ok = j < n;
+            src_ok = j < cn;
             zero_im = (g.mode0 + mode) == 0;   // int2d3d.f90:43 skips Im at k=0
-            if (src_ok) srcc = g.in[f] + (size_t)mode * m * m + (size_t)(j + 1) * m + 1;
+            if (src_ok) srcc = g.in[f] + (size_t)mode * fpl + (size_t)(j + 1) * ldf + 1;
         }
     }
 
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
         if constexpr (CIN) {
 #pragma unroll
             for (int e = 0; e < CPT; ++e) {
-                okc[e] = src_ok && k0 + e < n;
+                okc[e] = src_ok && k0 + e < kn;
                 creg[e] = srcc[okc[e] ? k0 + e : 0];
             }
         } else {
@@ -172,7 +175,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
                 double v = src_ok ? breg[e] : 0.0;
                 if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
                     const int k = ks * S::KT + kk0 + e;
-                    if (ld_j < n && k < n && !(ld_gauge && k == a.gauge)) v = v / (ld_lamj + lam_s[k] - ld_xlam);
+                    if (ld_j < cn && k < kn && !(ld_gauge && k == a.gauge)) v = v / (ld_lamj + lam_s[k] - ld_xlam);
                 }
                 B[v_dst * LDB + kk0 + e] = v;
             }
@@ -242,18 +245,18 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
             if (q >= a.ncols) continue;
             const long fm = q / NP;
             const int j = (int)(q % NP);
-            if (j >= n) continue;
+            if (j >= cn) continue;
             const int f = (int)(fm / g.nmodes), mode = (int)(fm % g.nmodes);
             const bool zim = (g.mode0 + mode) == 0;
-            d2v* out = g.out[f] + (size_t)mode * m * m + (j + 1);
+            d2v* out = g.out[f] + (size_t)mode * fpl + (j + 1);
 #pragma unroll
             for (int mt = 0; mt < TM; ++mt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = rbase + mt * 16 + 4 * r;
-                    if (row < n) {
+                    if (row < kn) {
                         d2v v = {acc[mt][2 * s][r], zim ? 0.0 : acc[mt][2 * s + 1][r]};
-                        out[(size_t)(row + 1) * m] = v;
+                        out[(size_t)(row + 1) * ldf] = v;
                     }
                 }
         }
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = rbase + mt * 16 + 4 * r;
-                    lrow[mt][r] = row < n ? lr[row] : 0.0;
+                    lrow[mt][r] = row < kn ? lr[row] : 0.0;
                 }
         }
 #pragma unroll
@@ -296,7 +299,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
                 const int f = (int)(fm / g.nmodes), mode = (int)(fm % g.nmodes);
                 xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
                 gauge_plane = a.gauge >= 0 && fabs(xlam) <= 1.0e-10;
-                lj = j < n ? a.lam[j] : 0.0;
+                lj = j < cn ? a.lam[j] : 0.0;
             }
 #pragma unroll
             for (int mt = 0; mt < TM; ++mt)
@@ -305,7 +308,7 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs a)
                     const int row = rbase + mt * 16 + 4 * r;
                     double v = acc[mt][nt][r];
                     if constexpr (MODE == PROJ_P2P_DIV) {
-                        if (j < n && row < n && !(gauge_plane && j == a.gauge && row == a.gauge))
+                        if (j < cn && row < kn && !(gauge_plane && j == a.gauge && row == a.gauge))
                             v = v / (lj + lrow[mt][r] - xlam);
                         else
                             v = 0.0;

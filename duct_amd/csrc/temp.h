@@ -6,16 +6,16 @@
 
 namespace duct {
 
-// Per-direction vectors of the walls, 8 m doubles:
-//   [0 m) D2y(:, 1)  [m 2m) D2y(:, m)  [2m 3m) D2z(:, 1)  [3m 4m) D2z(:, m)
-//   [4m 5m) Dy(1, :) [5m 6m) Dy(m, :)  [6m 7m) Dz(1, :)  [7m 8m) Dz(m, :)
+// Per-direction vectors of the walls, 4 my + 4 mz doubles:
+//   y: [0 my) D2y(:, 1), then D2y(:, my), Dy(1, :), Dy(my, :), my each
+//   z: from 4 my: D2z(:, 1), D2z(:, mz), Dz(1, :), Dz(mz, :), mz each
 struct TempArgs {
-    const d2v* rhs;        // complex fields (m, m, nmodes)
+    const d2v* rhs;        // complex fields (my, mz, nmodes), y fastest
     d2v* out;              // k_temp_rhs: corrected interior; k_temp_edges: solution (edges, corners)
-    const double* bcy;     // [mode][2][m] wall data over z (y = -1, y = +1)
-    const double* bcz;     // [mode][2][m] wall data over y (z = -1, z = +1)
-    const double* vec;     // the 8 m wall vectors above
-    int m, nmodes, mode0;
+    const double* bcy;     // [mode][2][mz] wall data over z (y = -1, y = +1)
+    const double* bcz;     // [mode][2][my] wall data over y (z = -1, z = +1)
+    const double* vec;     // the wall vectors above
+    int my, mz, nmodes, mode0;
     int by1, by2, bz1, bz2;   // 0 = Dirichlet, 1 = Neumann
 };
 // out interior = rhs interior + bcontv + bconth (mod_term_FTy / mod_term_FTz), Re and Im alike

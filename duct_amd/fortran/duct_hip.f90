@@ -218,19 +218,21 @@ module duct_hip
     end function
 
     ! mixed-boundary (temperature) solve: pseudomat_temp uploads and helm3d_temp
-    integer(c_int) function duct_setup_temp(bounds, m, PY, PYinv, ETY, DY, PZ, PZinv, ETZ, DZ, bounds_len) &
+    ! (my may differ from mz: PY, PYinv my-2 square, PZ, PZinv mz-2 square; DY my x my, DZ mz x mz)
+    integer(c_int) function duct_setup_temp(bounds, my, mz, PY, PYinv, ETY, DY, PZ, PZinv, ETZ, DZ, bounds_len) &
         bind(C, name="duct_setup_temp_")
       import :: c_int, c_double, c_size_t, c_char
       character(kind=c_char), intent(in) :: bounds(*)
-      integer(c_int), intent(in) :: m
+      integer(c_int), intent(in) :: my, mz
       real(c_double), intent(in) :: PY(*), PYinv(*), ETY(*), DY(*), PZ(*), PZinv(*), ETZ(*), DZ(*)
       integer(c_size_t), value :: bounds_len
     end function
-    integer(c_int) function duct_helm3d_temp_dev(rhs, sol, bcy, bcz, m, nmodes, ib, alp, alambda) &
+    ! rhs, sol complex(8) (my, mz, nmodes); bcy real(8) (mz, 2, nmodes), bcz (my, 2, nmodes)
+    integer(c_int) function duct_helm3d_temp_dev(rhs, sol, bcy, bcz, my, mz, nmodes, ib, alp, alambda) &
         bind(C, name="duct_helm3d_temp_dev_")
       import :: c_int, c_size_t, c_double
       integer(c_size_t), intent(in) :: rhs, sol, bcy, bcz
-      integer(c_int), intent(in) :: m, nmodes, ib
+      integer(c_int), intent(in) :: my, mz, nmodes, ib
       real(c_double), intent(in) :: alp, alambda
     end function
 

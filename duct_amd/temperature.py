@@ -3,12 +3,14 @@
 The host mirror of pseudomaty_temp / pseudomatz_temp (src/pseudomat_temp.f90:2-321) and
 helm3d_temp (src/helm3d_temp.f90:2-44):
 
-  setup_temp(m, bounds, sy, sz)   builds both directions' bases in FP64 on the host (the
+  setup_temp(m, bounds, sy, sz, mz)
+                                  builds both directions' bases in FP64 on the host (the
                                   reference's operation order for the wall elimination,
-                                  LAPACK through numpy) and uploads them (duct_setup_temp_)
+                                  LAPACK through numpy), y of size my = m and z of size mz
+                                  (default m), and uploads them (duct_setup_temp_)
   helm3d_temp(rhs, sol, bcy, bcz, alambda, alp, ib)
-                                  duct_helm3d_temp_dev_: sol = the solution with the wall
-                                  data bcy (nmodes, 2, m) over z and bcz (nmodes, 2, m) over y
+                                  duct_helm3d_temp_dev_: rhs, sol (nmodes, mz, my), the wall
+                                  data bcy (nmodes, 2, mz) over z and bcz (nmodes, 2, my) over y
 
 bounds is four characters 'd' / 'n' for the walls y=-1, y=+1, z=-1, z=+1 (the module
 variables boundy1, boundy2, boundz1, boundz2 of the reference).
@@ -20,7 +22,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .solver import _check_field, _use_current_stream
+from .solver import _use_current_stream
 
 
 def cheb_derivative(m: int, s: float) -> np.ndarray:
@@ -68,32 +70,38 @@ def pseudomat_temp(bound1: str, bound2: str, m: int, s: float):
 _state = {"temp": None}
 
 
-def setup_temp(m: int, bounds: str, sy: float = 1.0, sz: float = 1.0) -> None:
+def setup_temp(m: int, bounds: str, sy: float = 1.0, sz: float = 1.0, mz: int | None = None) -> None:
     """pseudomaty_temp(boundy1, boundy2, my) and pseudomatz_temp(boundz1, boundz2, mz), sy = a,
-    sz = b of helm3d_temp's ipro <= 0 branch, uploaded with duct_setup_temp_."""
+    sz = b of helm3d_temp's ipro <= 0 branch, uploaded with duct_setup_temp_; my = m."""
     if len(bounds) != 4 or any(c not in "dn" for c in bounds):
         raise ValueError("bounds = four of 'd' / 'n' for y=-1, y=+1, z=-1, z=+1")
+    mz = m if mz is None else int(mz)
     Dy, Py, Pyi, Ey = pseudomat_temp(bounds[0], bounds[1], m, sy)
-    Dz, Pz, Pzi, Ez = pseudomat_temp(bounds[2], bounds[3], m, sz)
+    Dz, Pz, Pzi, Ez = pseudomat_temp(bounds[2], bounds[3], mz, sz)
     cm = [np.asfortranarray(x) for x in (Py, Pyi, Dy, Pz, Pzi, Dz)]
     ptr = [x.ctypes.data_as(ctypes.c_void_p) for x in cm]
-    _lib.call("duct_setup_temp_", bounds.encode(), _lib.byref_int(m), ptr[0], ptr[1],
+    _lib.call("duct_setup_temp_", bounds.encode(), _lib.byref_int(m), _lib.byref_int(mz), ptr[0], ptr[1],
               Ey.ctypes.data_as(ctypes.c_void_p), ptr[2], ptr[3], ptr[4], Ez.ctypes.data_as(ctypes.c_void_p),
               ptr[5], ctypes.c_size_t(4))
-    _state["temp"] = dict(m=m, bounds=bounds, sy=sy, sz=sz)
+    _state["temp"] = dict(my=m, mz=mz, bounds=bounds, sy=sy, sz=sz)
 
 
 def helm3d_temp(rhs, sol, bcy, bcz, alambda: float, alp: float = 1.0, ib: int = 0) -> None:
-    """sol = helm3d_temp(rhs) on device tensors: rhs, sol complex128 (nmodes, m, m) [mode, z, y];
-    bcy, bcz float64 (nmodes, 2, m) [mode, wall, z] and [mode, wall, y]."""
-    m = rhs.shape[-1]
-    _check_field(rhs, m)
-    _check_field(sol, m)
-    M = rhs.shape[0]
-    for b in (bcy, bcz):
-        if tuple(b.shape) != (M, 2, m) or not b.is_contiguous() or str(b.dtype) != "torch.float64":
-            raise ValueError("bcy / bcz must be contiguous float64 (nmodes, 2, m)")
+    """sol = helm3d_temp(rhs) on device tensors: rhs, sol complex128 (nmodes, mz, my) [mode, z, y];
+    bcy float64 (nmodes, 2, mz) [mode, wall, z], bcz (nmodes, 2, my) [mode, wall, y]."""
+    import torch
+    for t in (rhs, sol):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex128):
+            raise TypeError("fields must be complex128 CUDA tensors")
+        if t.dim() != 3 or not t.is_contiguous():
+            raise ValueError("field must be contiguous (nmodes, mz, my) = [mode, z, y], got %s" % (tuple(t.shape),))
+    if sol.shape != rhs.shape:
+        raise ValueError("rhs and sol shapes differ")
+    M, mz, my = rhs.shape
+    for b, n, name in ((bcy, mz, "bcy"), (bcz, my, "bcz")):
+        if tuple(b.shape) != (M, 2, n) or not b.is_contiguous() or b.dtype != torch.float64 or not b.is_cuda:
+            raise ValueError("%s must be contiguous float64 CUDA (nmodes, 2, %d)" % (name, n))
     _use_current_stream()
     _lib.call("duct_helm3d_temp_dev_", *[_lib.byref_ptr(t.data_ptr()) for t in (rhs, sol, bcy, bcz)],
-              _lib.byref_int(m), _lib.byref_int(M), _lib.byref_int(ib), _lib.byref_double(alp),
-              _lib.byref_double(alambda))
+              _lib.byref_int(my), _lib.byref_int(mz), _lib.byref_int(M), _lib.byref_int(ib),
+              _lib.byref_double(alp), _lib.byref_double(alambda))

@@ -213,19 +213,21 @@ int duct_helm3d_shen_dev_(const duct_devptr_t *rhs, const duct_devptr_t *sol, co
 /* Mixed-boundary (temperature) solve, helm3d_temp (src/helm3d_temp.f90:2-44, int2d3d_temp.f90,
  * solvediag_temp.f90:2-352) on device fields.  duct_setup_temp_ is the GPU-upload tail of
  * pseudomaty_temp / pseudomatz_temp (src/pseudomat_temp.f90:2-321): bounds holds the four
- * wall types 'd'/'n' of y=-1, y=+1, z=-1, z=+1; PY, PYinv (PTY, PTYM1) and PZ, PZinv are the
- * interior (m-2)^2 eigenbases, ETY / ETZ their eigenvalues, DY / DZ the m x m derivative
- * matrices scaled by 1/sy, 1/sz (all column-major).  duct_helm3d_temp_dev_ solves
- * (D_yy + D_zz - xlambda) u = rhs, xlambda = alambda + (alp (ib+k))^2, for every mode k with
- * the wall data bcy (m, 2, nmodes) over z and bcz (m, 2, nmodes) over y (real(8); Dirichlet
- * walls take the value, Neumann walls the y- or z-derivative), used alike for Re and Im as
- * int2d3d_temp does.  sol's interior and edges are written, its corners are the rhs's, Im
- * is zero at k=0; rhs is not modified (the reference works on a copy).  my == mz only.    */
-int duct_setup_temp_(const char *bounds, const int *m, const double *PY, const double *PYinv, const double *ETY,
-                     const double *DY, const double *PZ, const double *PZinv, const double *ETZ, const double *DZ,
-                     size_t bounds_len);
+ * wall types 'd'/'n' of y=-1, y=+1, z=-1, z=+1; PY, PYinv (PTY, PTYM1) are the interior
+ * (my-2)^2 y eigenbasis, PZ, PZinv (PTZ, PTZM1) the (mz-2)^2 z eigenbasis, ETY / ETZ their
+ * eigenvalues, DY (my x my) / DZ (mz x mz) the derivative matrices scaled by 1/sy, 1/sz (all
+ * column-major); my and mz may differ (helm3d_temp.f90:35-36).  duct_helm3d_temp_dev_ solves
+ * (D_yy + D_zz - xlambda) u = rhs, xlambda = alambda + (alp (ib+k))^2, for every mode k of the
+ * complex(8) fields (my, mz, nmodes) with the wall data bcy (mz, 2, nmodes) over z and bcz
+ * (my, 2, nmodes) over y (real(8); Dirichlet walls take the value, Neumann walls the y- or
+ * z-derivative), used alike for Re and Im as int2d3d_temp does.  sol's interior and edges are
+ * written, its corners are the rhs's, Im is zero at k=0; rhs is not modified (the reference
+ * works on a copy).                                                                           */
+int duct_setup_temp_(const char *bounds, const int *my, const int *mz, const double *PY, const double *PYinv,
+                     const double *ETY, const double *DY, const double *PZ, const double *PZinv, const double *ETZ,
+                     const double *DZ, size_t bounds_len);
 int duct_helm3d_temp_dev_(const duct_devptr_t *rhs, const duct_devptr_t *sol, const duct_devptr_t *bcy,
-                          const duct_devptr_t *bcz, const int *m, const int *nmodes, const int *ib,
+                          const duct_devptr_t *bcz, const int *my, const int *mz, const int *nmodes, const int *ib,
                           const double *alp, const double *alambda);
 
 /* One substep krk (1..3) of the low-storage RK3 for velocity and pressure, the

@@ -18,7 +18,8 @@ Follows, in numpy (LAPACK dgeev / getrf / getri through numpy, dense products):
 
 Arrays: a plane U is [z, y] (C order) = the reference's U(my, mz) with y fastest; bcy is
 [wall, z] (the reference's bcy(mz, 2)), bcz is [wall, y]; per mode [mode, wall, .].
-Only my == mz is covered (the square grids the rest of the solver handles).
+my and mz may differ (helm3d_temp.f90:35-36 builds the y basis of size my and the z basis of
+size mz): a plane is then [mz, my], bcy [2, mz] and bcz [2, my].
 """
 from __future__ import annotations
 
@@ -70,12 +71,14 @@ def pseudomat_temp(bound1: str, bound2: str, m: int, s: float):
 class TempBasis:
     """The two directions' bases of helm3d_temp (pseudomaty_temp / pseudomatz_temp)."""
 
-    def __init__(self, m: int, bounds: str, sy: float = 1.0, sz: float = 1.0):
+    def __init__(self, m: int, bounds: str, sy: float = 1.0, sz: float = 1.0, mz: int | None = None):
+        """m = my, the y points; mz the z points (default my)."""
         if len(bounds) != 4 or any(c not in "dn" for c in bounds):
             raise ValueError("bounds = 4 characters of 'd' / 'n': y=-1, y=+1, z=-1, z=+1")
-        self.m, self.bounds, self.sy, self.sz = m, bounds, sy, sz
-        self.Dy, self.D2y, self.Py, self.Pyi, self.Ey = pseudomat_temp(bounds[0], bounds[1], m, sy)
-        self.Dz, self.D2z, self.Pz, self.Pzi, self.Ez = pseudomat_temp(bounds[2], bounds[3], m, sz)
+        mz = m if mz is None else mz
+        self.m, self.my, self.mz, self.bounds, self.sy, self.sz = m, m, mz, bounds, sy, sz
+        self.Dy, self.D2y, self.Py, self.Pyi, self.Ey = pseudomat_temp(bounds[0], bounds[1], m, sy)   # pseudomaty_temp
+        self.Dz, self.D2z, self.Pz, self.Pzi, self.Ez = pseudomat_temp(bounds[2], bounds[3], mz, sz)  # pseudomatz_temp
 
 
 def wall_terms(b1: str, b2: str, D: np.ndarray, bc: np.ndarray):
@@ -110,9 +113,10 @@ def edge_line(b1: str, b2: str, D: np.ndarray, bc: np.ndarray, U: np.ndarray):
 
 
 def solvediag_temp(tb: TempBasis, U: np.ndarray, xlambda: float, bcy: np.ndarray, bcz: np.ndarray):
-    """One real plane U [z, y] (rhs in, solution out, corners kept); bcy [2, m] over z, bcz [2, m]
+    """One real plane U [z, y] (rhs in, solution out, corners kept); bcy [2, mz] over z, bcz [2, my]
     over y.  Returns the new plane."""
-    m = tb.m
+    if U.shape != (tb.mz, tb.my):
+        raise ValueError("plane must be [mz, my]")
     by1, by2, bz1, bz2 = tb.bounds
     F = U[1:-1, 1:-1].copy()                                    # [z, y] interior
     gy1, gy2 = wall_terms(by1, by2, tb.Dy, bcy)                 # over z
@@ -137,7 +141,7 @@ def solvediag_temp(tb: TempBasis, U: np.ndarray, xlambda: float, bcy: np.ndarray
 
 def helm3d_temp(tb: TempBasis, rhs: np.ndarray, alambda: float, bcy: np.ndarray, bcz: np.ndarray,
                 alp: float = 1.0, ib: int = 0) -> np.ndarray:
-    """rhs [mode, z, y] complex; bcy [mode, 2, m] (over z), bcz [mode, 2, m] (over y), real.
+    """rhs [mode, z, y] complex; bcy [mode, 2, mz] (over z), bcz [mode, 2, my] (over y), real.
     Returns sol; rhs is not modified."""
     sol = np.zeros_like(rhs)
     for k in range(rhs.shape[0]):

@@ -1,7 +1,8 @@
 """The Fortran binding (duct_amd/fortran/duct_hip.f90) compiles with the image's
 flang and a program calling every bound entry point links against
-libduct_hip.so - CPU only.  The device entry points are only linked; the field-file
-reader, which is host code, is also run from Fortran on a fixture."""
+libduct_hip.so (CPU); the field-file reader, which is host code, is also run from
+Fortran on a fixture.  On a GPU a Fortran program runs the drop-in solve sequence
+and is checked against the oracle (test_fortran_program_solves_on_gpu)."""
 import os
 import shutil
 import subprocess
@@ -48,8 +49,9 @@ program use_duct
   call duct_check(duct_fourx_dev(r(1), s(1), m, m, 16_c_int, 48_c_int, 1_c_int), 'fourx')
   call duct_check(duct_rk3_substep_dev(r(1), r(1), r(1), r(1), s(1), s(1), s(1), m, 16_c_int, 48_c_int, &
        1.0_c_double, 1.0_c_double, 0.01_c_double, 0.1_c_double, 1_c_int, 0.0_c_double), 'rk3')
-  call duct_check(duct_setup_temp('dndn', m, P, Pinv, lam, DC, P, Pinv, lam, DC, 4_c_size_t), 'setup_temp')
-  call duct_check(duct_helm3d_temp_dev(r(1), s(1), r(1), r(1), m, nm, 0_c_int, 1.0_c_double, 2.0_c_double), 'temp')
+  call duct_check(duct_setup_temp('dndn', m, m, P, Pinv, lam, DC, P, Pinv, lam, DC, 4_c_size_t), 'setup_temp')
+  call duct_check(duct_helm3d_temp_dev(r(1), s(1), r(1), r(1), m, m, nm, 0_c_int, 1.0_c_double, 2.0_c_double), &
+       'temp')
   call duct_check(duct_setup_shen('d', m, P, P, P, P, lam, lam, lam, lam, 1_c_size_t), 'preshen')
   call duct_check(duct_helm3d_shen_dev(r(1), s(1), m, nm, 0_c_int, 1.0_c_double, 2.0_c_double, 'd', 1_c_size_t), &
        'shen')
@@ -135,3 +137,99 @@ def test_fortran_reads_a_field_file(tmp_path):
     # bcy(z=5, wall 2, mode 1) and bcy(z=9, wall 1, mode 0), 1-based z
     assert [float(x) for x in out[3].split()] == [100.0 + 8.0 + 4.0 + 4 * 0.125, 100.0 + 8 * 0.125]
     assert out[4].split() == ["1", "n"]
+
+
+SOLVER = """
+program solve_cpu33
+  ! the reference's sequence (main1.f90:149-150 device init, pseudomat.f90:188-221 and 265-298
+  ! uploads, helm3d.f90:48-73 solve) through the bind(C) module, on host arrays
+  use iso_c_binding
+  use duct_hip
+  implicit none
+  character(len=512) :: fin, fout
+  integer(c_int) :: m, nm, iofz(2)
+  real(c_double) :: wanted(2)
+  real(c_double), allocatable :: P(:, :, :), Pinv(:, :, :), lam(:, :), DC(:, :), AM1(:, :, :)
+  complex(c_double_complex), allocatable :: rhs(:, :, :, :), sol(:, :, :, :)
+  character(kind=c_char) :: bnd(2)
+  real(c_double) :: alam(2)
+  integer :: b
+  bnd = ['d', 'n']
+  alam = [1000.0_c_double, 0.0_c_double]
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(10) m, nm
+  allocate(P(m-2, m-2, 2), Pinv(m-2, m-2, 2), lam(m-2, 2), DC(m, m), AM1(4, 4, 2))
+  allocate(rhs(m, m, nm, 2), sol(m, m, nm, 2))
+  read(10) DC
+  do b = 1, 2
+    read(10) P(:, :, b), Pinv(:, :, b), lam(:, b), AM1(:, :, b), iofz(b), wanted(b)
+  end do
+  read(10) rhs
+  close(10)
+  call duct_check(duct_init(0_c_int), 'init')
+  do b = 1, 2
+    call duct_check(duct_setup(bnd(b), m, P(:, :, b), Pinv(:, :, b), lam(:, b), DC, AM1(:, :, b), iofz(b), &
+                               wanted(b), 1_c_size_t), 'setup')
+  end do
+  do b = 1, 2
+    call duct_check(duct_helm3d(rhs(:, :, :, b), sol(:, :, :, b), m, nm, 0_c_int, 1.0_c_double, alam(b), &
+                                bnd(b), 1_c_size_t), 'helm3d')
+  end do
+  call duct_check(duct_shutdown(), 'shutdown')
+  open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(11) sol
+  close(11)
+  write(*, '(a)') 'solved'
+end program solve_cpu33
+"""
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(FC is None, reason="no Fortran compiler in this image")
+def test_fortran_program_solves_on_gpu(tmp_path, oracle_mod):
+    """A flang-built program runs the drop-in sequence on the GPU: duct_init, duct_setup for 'd' and
+    'n' with the oracle's own bases (what the reference's pseudomat computes and hands over), then
+    duct_helm3d on the 33^2 x 16 CPU workload's right-hand sides (u, Dirichlet, alambda 1000, and p,
+    Neumann, alambda 0).  Both solutions match the oracle's helm3d to the FP64 bar."""
+    import numpy as np
+    from duct_amd import workloads
+    lib = os.path.join(ROOT, "duct_amd", "libduct_hip.so")
+    if not os.path.exists(lib):
+        pytest.skip("libduct_hip.so not built")
+    mod = os.path.join(ROOT, "duct_amd", "fortran", "duct_hip.f90")
+    prog = tmp_path / "solve_cpu33.f90"
+    prog.write_text(SOLVER)
+    subprocess.run([FC, "-c", mod, "-J", str(tmp_path), "-o", str(tmp_path / "duct_hip.o")], check=True,
+                   cwd=tmp_path)
+    subprocess.run([FC, str(prog), str(tmp_path / "duct_hip.o"), "-I", str(tmp_path), "-L",
+                    os.path.dirname(lib), "-lduct_hip", "-L/opt/rocm/lib", "-lamdhip64",
+                    "-Wl,-rpath," + os.path.dirname(lib), "-o", str(tmp_path / "solve_cpu33")],
+                   check=True, cwd=tmp_path)
+    cfg = workloads.CPU_33
+    m, nm = cfg.m, cfg.nmodes
+    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(m))
+    b = ob.b
+    f = [workloads.rhs_field(cfg, 0), workloads.rhs_field(cfg, 3)]     # u ('d') and p ('n')
+    assert workloads.BOUNDS[0] == "d" and workloads.BOUNDS[3] == "n"
+
+    def fa(a):
+        return np.asarray(a, dtype=np.float64).tobytes(order="F")
+    blob = [np.array([m, nm], dtype=np.int32).tobytes(), fa(b.DC)]
+    for P, Pi, lam, am1, io, wa in ((b.PD, b.PDM1, b.ED, np.zeros((4, 4)), 0, 0.0),
+                                    (b.PN, b.PNM1, b.EN, b.AM1, b.iofzero, b.wanted)):
+        blob += [fa(P), fa(Pi), fa(lam), fa(am1), np.array([io], dtype=np.int32).tobytes(), fa([wa])]
+    blob.append(np.ascontiguousarray(np.stack(f)).tobytes())          # Fortran rhs(m, m, nm, 2)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    fin.write_bytes(b"".join(blob))
+    print("running the Fortran drop-in program, 33^2 x 16, d and n", flush=True)
+    out = subprocess.run([str(tmp_path / "solve_cpu33"), str(fin), str(fout)], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    got = np.frombuffer(fout.read_bytes(), dtype=np.complex128).reshape(2, nm, m, m)
+    for i, (bound, alam) in enumerate((("d", 1000.0), ("n", 0.0))):
+        ref, _ = oracle_mod.helm3d(ob, f[i], bound, alam, alp=1.0, ib=0)
+        err = float(np.max(np.abs(got[i] - ref)) / np.max(np.abs(ref)))
+        print("Fortran drop-in %s: max rel err vs oracle %.2e" % (bound, err), flush=True)
+        assert err <= 1e-12, (bound, err)

@@ -18,15 +18,20 @@ from numpy.polynomial import chebyshev as C
 from oracle import temp as T
 
 
-def manufactured(m, sy, sz, xlam, seed, deg=None):
+def manufactured(m, sy, sz, xlam, seed, deg=None, mz=None):
+    """u on the [mz, my] grid (my = m), degree <= deg in y and <= deg + mz - m in z."""
+    mz = m if mz is None else mz
     deg = m - 3 if deg is None else deg
+    degz = deg + mz - m
     rng = np.random.default_rng(seed)
-    A = rng.standard_normal((deg + 1, deg + 1)) * np.exp(-0.3 * np.add.outer(np.arange(deg + 1), np.arange(deg + 1)))
-    x = -np.cos(np.pi * np.arange(m) / (m - 1))
+    A = rng.standard_normal((degz + 1, deg + 1)) * np.exp(-0.3 * np.add.outer(np.arange(degz + 1),
+                                                                             np.arange(deg + 1)))
+    xy = -np.cos(np.pi * np.arange(m) / (m - 1))
+    xz = -np.cos(np.pi * np.arange(mz) / (mz - 1))
 
     def ev(Ac):                                    # Ac[q, p] -> values [z, y] at the nodes
-        Ty = np.stack([C.chebval(x, np.eye(Ac.shape[1])[p]) for p in range(Ac.shape[1])], 1)
-        Tz = np.stack([C.chebval(x, np.eye(Ac.shape[0])[q]) for q in range(Ac.shape[0])], 1)
+        Ty = np.stack([C.chebval(xy, np.eye(Ac.shape[1])[p]) for p in range(Ac.shape[1])], 1)
+        Tz = np.stack([C.chebval(xz, np.eye(Ac.shape[0])[q]) for q in range(Ac.shape[0])], 1)
         return Tz @ Ac @ Ty.T
 
     u = ev(A)
@@ -58,6 +63,27 @@ def test_manufactured_solution_every_wall_combination(bounds):
     assert err <= 1e-10, (bounds, err)
     for (a, b) in ((0, 0), (0, -1), (-1, 0), (-1, -1)):
         assert got[a, b] == f[a, b]                                   # corners untouched
+
+
+@pytest.mark.parametrize("my,mz", [(13, 21), (21, 13), (9, 33)])
+@pytest.mark.parametrize("bounds", ["dndn", "nddn", "nnnn", "dddd"])
+def test_manufactured_solution_rectangular_grid(my, mz, bounds):
+    """my != mz: the y basis has my points, the z basis mz (helm3d_temp.f90:35-36); a transposed
+    operand or a swapped size in any step fails these."""
+    sy, sz, xlam = 1.0, 2.0, 1.5
+    tb = T.TempBasis(my, bounds, sy, sz, mz=mz)
+    assert tb.Py.shape == (my - 2, my - 2) and tb.Pz.shape == (mz - 2, mz - 2)
+    u, uy, uz, f = manufactured(my, sy, sz, xlam, my * 100 + mz, mz=mz)
+    assert f.shape == (mz, my)
+    bcy, bcz = wall_data(bounds, u, uy, uz)
+    assert bcy.shape == (2, mz) and bcz.shape == (2, my)
+    got = T.solvediag_temp(tb, f, xlam, bcy, bcz)
+    inner = np.ones((mz, my), bool)
+    inner[0, 0] = inner[0, -1] = inner[-1, 0] = inner[-1, -1] = False
+    err = np.abs(got - u)[inner].max() / np.abs(u).max()
+    assert err <= 1e-10, (my, mz, bounds, err)
+    with pytest.raises(ValueError):
+        T.solvediag_temp(tb, f.T.copy(), xlam, bcy, bcz)
 
 
 def test_helm3d_temp_modes_and_rhs():
