@@ -313,8 +313,12 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc) {
             h8v (&bc)[2][TI] = bq[kc & 1];
-            if (kc + 1 < KC) load_b(pass, kc + 1, bq[(kc + 1) & 1]);
-            else if (pass < 3) load_b(pass + 1, 0, bq[(kc + 1) & 1]);
+            // the next chunk, unconditionally (after pass 3's last chunk a harmless reload of pass
+            // 0's first): a load under a branch would make every outstanding load be waited for
+            load_b(kc + 1 < KC ? pass : (pass + 1) & 3, (kc + 1) % KC, bq[(kc + 1) & 1]);
+            // keep the chunk's products below these loads and above the next chunk's, so that the
+            // loads have a whole chunk of products to land behind
+            __builtin_amdgcn_sched_barrier(0);
             // one j tile's A fragments at a time (the registers go to the accumulator); the three
             // products into am[x][y] are TI launches apart, not back to back
 #pragma unroll
@@ -333,6 +337,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
                 for (int y = 0; y < TI; ++y)
                     am[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bc[0][y], am[x][y], 0, 0, 0);
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
         // KC is even: the next pass's first chunk sits in bq[0]
         static_assert(KC % 2 == 0, "the B chunk double buffer assumes an even chunk count");
@@ -340,22 +345,47 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
         // together with the next pass's scale; D[j][i]: lane i + 16 (j/4), element j%4
         inv = 1.f / (sx * sb);
         mx = 0.f;
+        if (pass == 1) {   // G = ... ./ (lam_r(j) + lam_c(i) - xlambda), solvediag.f90:61-66, :151-159
+            // one uniform branch for the tile; lam_r - xlambda per row and lam_c per column read
+            // once, +inf past the parity's size so that rcp gives 0 on the padding
+            float lr[TJ][4], lc[TI];
+#pragma unroll
+            for (int x = 0; x < TJ; ++x)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int j = j0 + x * 16 + l4 * 4 + e;
+                    lr[x][e] = j < hh[r] ? lam_s[r * NH + j] - xlam : __builtin_huge_valf();
+                }
+#pragma unroll
+            for (int y = 0; y < TI; ++y) {
+                const int i = i0 + y * 16 + l16;
+                lc[y] = i < hh[c] ? lam_s[c * NH + i] : __builtin_huge_valf();
+            }
+#pragma unroll
+            for (int x = 0; x < TJ; ++x)
+#pragma unroll
+                for (int y = 0; y < TI; ++y)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)   // rcp: ~1 ulp, inside the split's 2^-22
+                        am[x][y][e] *= __builtin_amdgcn_rcpf(lr[x][e] + lc[y]);
+            if (gauge) {   // the singular plane's null-mode entry (den = 0) is 0
+#pragma unroll
+                for (int x = 0; x < TJ; ++x)
+#pragma unroll
+                    for (int y = 0; y < TI; ++y)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int j = j0 + x * 16 + l4 * 4 + e, i = i0 + y * 16 + l16;
+                            if (j == a.gauge && i == a.gauge) am[x][y][e] = 0.f;
+                        }
+            }
+        }
 #pragma unroll
         for (int x = 0; x < TJ; ++x)
 #pragma unroll
             for (int y = 0; y < TI; ++y)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    float t = am[x][y][e];
-                    if (pass == 1) {   // G = ... ./ (lam_r(j) + lam_c(i) - xlambda), solvediag.f90:61-66, :151-159
-                        const int j = j0 + x * 16 + l4 * 4 + e, i = i0 + y * 16 + l16;
-                        const bool ok = j < hh[r] && i < hh[c] && !(gauge && j == a.gauge && i == a.gauge);
-                        const float den = lam_s[r * NH + j] + lam_s[c * NH + i] - xlam;
-                        t = ok ? t * __builtin_amdgcn_rcpf(den) : 0.f;   // ~1 ulp, inside the split's 2^-22
-                    }
-                    am[x][y][e] = t;
-                    mx = fmaxf(mx, fabsf(t));
-                }
+                for (int e = 0; e < 4; ++e) mx = fmaxf(mx, fabsf(am[x][y][e]));
         mx *= inv;
         if (pass == 3) break;
         sx = scale_of(block_max(mx));   // the barrier inside: every wave is done reading Xs
