@@ -163,6 +163,12 @@ struct Lib {
     hipStream_t side = nullptr;
     double* plan2[2] = {nullptr, nullptr};
     size_t plan2_bytes = 0;
+    float* mfold2 = nullptr;              // and its folded mixed-solve arrays, digits, residual fields
+    size_t mfold2_bytes = 0;
+    unsigned char* mdig2 = nullptr;
+    size_t mdig2_bytes = 0;
+    d2v* rfield2 = nullptr;
+    size_t rfield2_bytes = 0;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int overlap = 1;
 };
@@ -869,6 +875,30 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
     return OK;
 }
 
+// While alive, the library's stream and work buffers (planar, folded, digit and residual arrays) are
+// the side stream's (the Neumann group of a batched solve runs there, concurrently with the
+// Dirichlet group on the caller's stream).
+struct SideLane {
+    SideLane() { swap(); }
+    ~SideLane() { swap(); }
+    static void swap()
+    {
+        std::swap(L.stream, L.side);
+        std::swap(L.plan[0], L.plan2[0]);
+        std::swap(L.plan[1], L.plan2[1]);
+        std::swap(L.plan_bytes, L.plan2_bytes);
+        std::swap(L.mfold, L.mfold2);
+        std::swap(L.mfold_bytes, L.mfold2_bytes);
+        std::swap(L.mdig, L.mdig2);
+        std::swap(L.mdig_bytes, L.mdig2_bytes);
+        std::swap(L.rfield, L.rfield2);
+        std::swap(L.rfield_bytes, L.rfield2_bytes);
+    }
+};
+
+int solve_group_mixed(const FieldGroup& g, int bidx, bool folded, bool i8, int scheme0, int scheme1, int niter,
+                      int nstat, int nf, const int* fidx);
+
 int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* sol, const char* bounds,
                        const double* alambda, int m, int nmodes, int ib, double alp, int scheme0, int scheme1,
                        int niter, double* stats)
@@ -893,16 +923,17 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
         if ((st = ensure_buf(L.stats, L.stats_bytes, sizeof(unsigned long long) * 2 * nf * nstat))) return st;
         HIPCHK(hipMemsetAsync(L.stats, 0, sizeof(unsigned long long) * 2 * nf * nstat, L.stream));
     }
+    FieldGroup gs[2]{};
+    int fidxs[2][kMaxFields];
     for (int bidx = 0; bidx < 2; ++bidx) {
-        FieldGroup g{};
+        FieldGroup& g = gs[bidx];
         g.nmodes = nmodes; g.mode0 = ib; g.m = m; g.n = m - 2; g.alp = alp;
-        int fidx[kMaxFields];
         for (int f = 0; f < nf; ++f) {
             const int bi = bound_index(bounds[f]);
             if (bi < 0) return fail(ERR_ARG, "bound '%c' of field %d is neither 'd' nor 'n'", bounds[f], f);
             if (bi != bidx) continue;
             if (!rhs[f] || !sol[f]) return fail(ERR_ARG, "null device pointer for field %d", f);
-            fidx[g.nf] = f;
+            fidxs[bidx][g.nf] = f;
             g.in[g.nf] = (const d2v*)rhs[f];
             g.out[g.nf] = (d2v*)sol[f];
             g.alambda[g.nf] = alambda[f];
@@ -912,13 +943,57 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
         DevBasis& B = L.basis[bidx];
         if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
         if (g.m != B.m) return fail(ERR_ARG, "m=%d but the basis was set up for m=%d", g.m, B.m);
-        if (folded) {
-            if (!B.lq_ready)
-                return fail(ERR_UNSUPPORTED, "scheme 5 needs a folding basis with ceil((m-2)/2) in (32, 128] "
-                            "(m=%d, bound '%c')", m, bidx ? 'n' : 'd');
-            if ((st = run_group_mixed_fold(g, bidx, niter, nstat ? L.stats : nullptr, fidx, nf, i8))) return st;
-            continue;
+        if (folded && !B.lq_ready)
+            return fail(ERR_UNSUPPORTED, "scheme 5 needs a folding basis with ceil((m-2)/2) in (32, 128] "
+                        "(m=%d, bound '%c')", m, bidx ? 'n' : 'd');
+    }
+    // The folded solve with both groups: the Neumann group on the side stream with its own work
+    // arrays, so that its latency-bound kernels (null projection, low-mode correction, gauge,
+    // edges) fill the gaps of the Dirichlet group's (not while launches are timed).
+    if (folded && gs[0].nf && gs[1].nf && L.overlap && !L.prof) {
+        if (!L.side) {
+            HIPCHK(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&L.ev_fork, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&L.ev_join, hipEventDisableTiming));
         }
+        HIPCHK(hipEventRecord(L.ev_fork, L.stream));
+        HIPCHK(hipStreamWaitEvent(L.side, L.ev_fork, 0));
+        st = run_group_mixed_fold(gs[0], 0, niter, nstat ? L.stats : nullptr, fidxs[0], nf, i8);
+        int st_side;
+        {
+            SideLane lane;
+            st_side = run_group_mixed_fold(gs[1], 1, niter, nstat ? L.stats : nullptr, fidxs[1], nf, i8);
+        }
+        HIPCHK(hipEventRecord(L.ev_join, L.side));
+        HIPCHK(hipStreamWaitEvent(L.stream, L.ev_join, 0));
+        if (st || st_side) return st ? st : st_side;
+    } else {
+        for (int bidx = 0; bidx < 2; ++bidx) {
+            if (gs[bidx].nf == 0) continue;
+            if ((st = solve_group_mixed(gs[bidx], bidx, folded, i8, scheme0, scheme1, niter, nstat, nf,
+                                        fidxs[bidx])))
+                return st;
+        }
+    }
+    if (nstat) {
+        std::vector<unsigned long long> h((size_t)2 * nf * nstat);
+        HIPCHK(hipMemcpyAsync(h.data(), L.stats, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              L.stream));
+        HIPCHK(hipStreamSynchronize(L.stream));
+        for (size_t e = 0; e < h.size(); ++e) stats[e] = __builtin_bit_cast(double, h[e]);
+    }
+    return OK;
+}
+
+// One boundary group of solve_fields_mixed, on the library's current stream.
+int solve_group_mixed(const FieldGroup& g, int bidx, bool folded, bool i8, int scheme0, int scheme1, int niter,
+                      int nstat, int nf, const int* fidx)
+{
+    DevBasis& B = L.basis[bidx];
+    const int m = g.m, nmodes = g.nmodes;
+    int st;
+    {
+        if (folded) return run_group_mixed_fold(g, bidx, niter, nstat ? L.stats : nullptr, fidx, nf, i8);
         if ((st = run_group_lp(g, bidx, scheme0, 0, B.wanted))) return st;
         ResidArgs ra{};
         ra.g = g;
@@ -971,13 +1046,6 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
             HIPCHK(run_boundary(ba));
         }
     }
-    if (nstat) {
-        std::vector<unsigned long long> h((size_t)2 * nf * nstat);
-        HIPCHK(hipMemcpyAsync(h.data(), L.stats, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                              L.stream));
-        HIPCHK(hipStreamSynchronize(L.stream));
-        for (size_t e = 0; e < h.size(); ++e) stats[e] = __builtin_bit_cast(double, h[e]);
-    }
     return OK;
 }
 
@@ -989,20 +1057,6 @@ int check_common(int m, int nmodes)
 }
 
 // Solve a batch of device fields: group by boundary type, one run per group.
-// While alive, the library's stream and planar buffers are the side stream's (the Neumann group of
-// a batched solve runs there, concurrently with the Dirichlet group on the caller's stream).
-struct SideLane {
-    SideLane() { swap(); }
-    ~SideLane() { swap(); }
-    static void swap()
-    {
-        std::swap(L.stream, L.side);
-        std::swap(L.plan[0], L.plan2[0]);
-        std::swap(L.plan[1], L.plan2[1]);
-        std::swap(L.plan_bytes, L.plan2_bytes);
-    }
-};
-
 int solve_fields(int nf, const duct_devptr_t* rhs, const duct_devptr_t* sol, const char* bounds,
                  const double* alambda, int m, int nmodes, int ib, double alp, int dir_bmode)
 {
@@ -1157,6 +1211,10 @@ int duct_shutdown_(void)
     for (auto& p : L.plan) free_ptr(p);
     for (auto& p : L.plan2) free_ptr(p);
     L.plan2_bytes = 0;
+    free_ptr(L.mfold2);
+    free_ptr(L.mdig2);
+    free_ptr(L.rfield2);
+    L.mfold2_bytes = L.mdig2_bytes = L.rfield2_bytes = 0;
     if (L.ev_fork) (void)hipEventDestroy(L.ev_fork);
     if (L.ev_join) (void)hipEventDestroy(L.ev_join);
     if (L.ev_switch) (void)hipEventDestroy(L.ev_switch);
