@@ -653,7 +653,7 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
     const int NH = B.NH;
     const size_t arr = (size_t)g.nf * g.nmodes * 2 * 4 * NH * NH;   // floats per folded array
     // F, R, U0 (FP64-residual form) or Flo (int8 form: U0 is in digits), D1..Dniter, Dlow
-    const int narr = 2 + 1 + niter + 1 + (i8 ? 1 : 0);                // + the rounded U0 (int8 form)
+    const int narr = 2 + 1 + niter + 1;
     int st = ensure_buf(L.mfold, L.mfold_bytes, arr * narr * sizeof(float));
     if (st) return st;
     float* F = L.mfold;
@@ -661,7 +661,6 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
     float* D[kMaxFoldSources + 1];
     for (int t = 0; t < niter + 2; ++t) D[t] = L.mfold + (size_t)(2 + t) * arr;   // D[0] = U0, D[niter+1] = low
     float* Flo = i8 ? D[0] : nullptr;
-    float* U0q = i8 ? L.mfold + (size_t)(narr - 1) * arr : nullptr;
     unsigned char* Ucol = nullptr;
     unsigned char* Urow = nullptr;
     int* gexp = nullptr;
@@ -737,14 +736,14 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
         // U0 in digit form, the residual on the int8 matrix cores, one correction (niter == 1)
         {
             Timed t(PK_LP_P2P);
-            la.g = g; la.X = F; la.Y = U0q;
+            la.g = g; la.X = F; la.Y = nullptr;
             la.Ucol = Ucol; la.Urow = Urow; la.gexp = gexp;
             HIPCHK(launch_lpq(NH, la, L.stream));
         }
         ResidI8Args ra{};
         ra.g = g;
         ra.Fhi = F; ra.Flo = Flo;
-        ra.Ucol = Ucol; ra.Urow = Urow; ra.gexp = gexp; ra.U0 = U0q;
+        ra.Ucol = Ucol; ra.Urow = Urow; ra.gexp = gexp;
         ra.R = R;
         ra.Sdig[0] = B.sdig[0]; ra.Sdig[1] = B.sdig[1];
         ra.Sexp[0] = B.sexp[0]; ra.Sexp[1] = B.sexp[1];
@@ -760,8 +759,8 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
         if (bidx == 1 && (st = nullproj(g, sp, 0))) return st;
         if ((st = lpq(g, R, D[1]))) return st;
         Timed t(PK_LP_P2C);
-        const float* src[kMaxFoldSources] = {U0q, D[1], nullptr, nullptr};
-        HIPCHK(launch_unfold_sum(g, NH, src, 2, 0, L.stream));
+        const float* src[kMaxFoldSources] = {D[1], nullptr, nullptr, nullptr};
+        HIPCHK(launch_unfold_sum(g, NH, src, 1, 0, L.stream, Ucol, gexp, bidx == 0));
     } else {
         if ((st = lpq(g, F, D[0]))) return st;
         for (int it = 0; it < niter; ++it) {
@@ -770,7 +769,7 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
         }
         Timed t(PK_LP_P2C);
         const float* src[kMaxFoldSources] = {D[0], niter >= 1 ? D[1] : nullptr, niter >= 2 ? D[2] : nullptr, nullptr};
-        HIPCHK(launch_unfold_sum(g, NH, src, niter + 1, 0, L.stream));
+        HIPCHK(launch_unfold_sum(g, NH, src, niter + 1, 0, L.stream, nullptr, nullptr, bidx == 0));
     }
     // The refinement above converges to S^-1 f, S the centrosymmetric part of D2B.  On the
     // ill-conditioned low Poisson modes (xlambda < kLowModes) that differs from D2B^-1 f by up
@@ -865,11 +864,11 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
         }
     }
     if (stats && (st = resid_full(g, stats + (size_t)2 * nf_all * niter))) return st;
-    {
+    if (bidx == 1) {   // (the Dirichlet boundary was zeroed by the unfold)
         Timed t(PK_BOUNDARY);
         BoundaryArgs ba{};
         ba.g = g; ba.DC = B.DC; ba.AM1 = B.AM1;
-        ba.mode = bidx == 1 ? BND_NEUMANN : BND_DIR_ZERO;
+        ba.mode = BND_NEUMANN;
         HIPCHK(run_boundary(ba));
     }
     return OK;
@@ -1405,12 +1404,38 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
         L.dc_m = m;
         L.dc_np = MP;
     }
+    // the interior operator D2B of the solve (pseudomat.f90:110-128, :225-240), column-major
+    std::vector<double> op((size_t)n * n, 0.0);
+    {
+        std::vector<double> D2((size_t)m * m, 0.0);
+        for (int j = 0; j < m; ++j)          // D2 = DC * DC (pseudomat.f90:110)
+            for (int k = 0; k < m; ++k) {
+                const double b = DC[k + (size_t)j * m];
+                for (int i = 0; i < m; ++i) D2[i + (size_t)j * m] += DC[i + (size_t)k * m] * b;
+            }
+        const int N = m - 1;
+        const double Det = DC[0] * DC[N + (size_t)N * m] - DC[(size_t)N * m] * DC[N];
+        for (int jj = 0; jj < n; ++jj)
+            for (int ii = 0; ii < n; ++ii) {
+                const int i = ii + 1, j = jj + 1;
+                double v = D2[i + (size_t)j * m];               // Dirichlet: pseudomat.f90:232-240
+                if (bidx == 1) {                                 // Neumann: :114-128
+                    const double ter1 = D2[i] * (DC[(size_t)N * m] * DC[N + (size_t)j * m] -
+                                                 DC[N + (size_t)N * m] * DC[(size_t)j * m]) / Det;
+                    const double ter3 = D2[i + (size_t)N * m] * (DC[N] * DC[(size_t)j * m] -
+                                                                 DC[0] * DC[N + (size_t)j * m]) / Det;
+                    v = ter1 + v + ter3;
+                }
+                op[ii + (size_t)jj * n] = v;
+            }
+    }
     {   // even/odd folded bases (used when duct_set_fold_ enables them)
         const int NH = fold_size(n);
         if (NH > 0) {
             std::vector<double> ff((size_t)2 * NH * NH), fb((size_t)2 * NH * NH), fl((size_t)2 * NH);
             std::vector<int> pos(n);
-            bool ok = build_fold_bases(P, Pinv, lam, n, NH, ff.data(), fb.data(), fl.data(), pos.data()) == 0;
+            bool ok = build_fold_bases(P, Pinv, lam, n, NH, ff.data(), fb.data(), fl.data(), pos.data(),
+                                       op.data()) == 0;
             std::vector<double> fprow((size_t)2 * NH, 0.0);
             if (ok && bidx == 1) {   // the null mode must be even; the gauge node a = iofzero-1
                 const int io = *iofzero - 1, he = (n + 1) / 2;
@@ -1446,27 +1471,6 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
 
     // mixed-precision solve: low-precision bases and the FP64 interior operator
     {
-        std::vector<double> D2((size_t)m * m, 0.0), op((size_t)n * n, 0.0);
-        for (int j = 0; j < m; ++j)          // D2 = DC * DC (pseudomat.f90:110)
-            for (int k = 0; k < m; ++k) {
-                const double b = DC[k + (size_t)j * m];
-                for (int i = 0; i < m; ++i) D2[i + (size_t)j * m] += DC[i + (size_t)k * m] * b;
-            }
-        const int N = m - 1;
-        const double Det = DC[0] * DC[N + (size_t)N * m] - DC[(size_t)N * m] * DC[N];
-        for (int jj = 0; jj < n; ++jj)
-            for (int ii = 0; ii < n; ++ii) {
-                const int i = ii + 1, j = jj + 1;
-                double v = D2[i + (size_t)j * m];               // Dirichlet: pseudomat.f90:232-240
-                if (bidx == 1) {                                 // Neumann: :114-128
-                    const double ter1 = D2[i] * (DC[(size_t)N * m] * DC[N + (size_t)j * m] -
-                                                 DC[N + (size_t)N * m] * DC[(size_t)j * m]) / Det;
-                    const double ter3 = D2[i + (size_t)N * m] * (DC[N] * DC[(size_t)j * m] -
-                                                                 DC[0] * DC[N + (size_t)j * m]) / Det;
-                    v = ter1 + v + ter3;
-                }
-                op[ii + (size_t)jj * n] = v;
-            }
         std::vector<double> opf((size_t)NP * NP);
         build_frag(op.data(), n, n, NP, opf.data());
         if ((st = up(B.opfrag, opf.data(), opf.size()))) return st;
@@ -1474,7 +1478,7 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
             const int NH = B.NH, he = (n + 1) / 2, ho = n / 2;
             std::vector<double> Af((size_t)2 * NH * NH), Ab((size_t)2 * NH * NH), fl((size_t)2 * NH);
             std::vector<int> pos(n);
-            if (fold_matrices(P, Pinv, lam, n, NH, Af.data(), Ab.data(), fl.data(), pos.data()) == 0) {
+            if (fold_matrices(P, Pinv, lam, n, NH, Af.data(), Ab.data(), fl.data(), pos.data(), op.data()) == 0) {
                 auto upraw2 = [&](void*& dst, const void* src, size_t bytes) -> int {
                     HIPCHK(hipMalloc(&dst, bytes));
                     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));

@@ -19,9 +19,12 @@ constexpr int kMaxFoldSources = 4;
 // carries q to ~2^-48.
 hipError_t launch_fold32(const FieldGroup& g, int NH, float* X, hipStream_t st, float* Xlo = nullptr);
 
-// unfold: g.out interior (= or +=) the unfolded sum of nsrc folded FP32 arrays (summed in FP64).
+// unfold: g.out interior (= or +=) the unfolded sum of nsrc folded FP32 arrays (summed in FP64),
+// preceded, with udig non-null, by the digit-form array of launch_lpq (udig = Ucol, gexp);
+// zero_edges: the boundary of g.out and of g.in set to 0 as well (helm3d.f90:48-60).
 hipError_t launch_unfold_sum(const FieldGroup& g, int NH, const float* const* src, int nsrc, int accumulate,
-                             hipStream_t st);
+                             hipStream_t st, const unsigned char* udig = nullptr, const int* gexp = nullptr,
+                             int zero_edges = 0);
 
 // The fused low-precision solve of every quarter: U = P_r ((P_r^-1 X P_c^-T) ./ den) P_c^T
 // with fp16 hi/lo splits (3 products, FP32 accumulation), per-pass power-of-two scaling.
@@ -32,11 +35,10 @@ struct LpqArgs {
     const float* lam;       // [lam_e | lam_o], NH each
     const float* X;         // folded FP32 input
     float* Y;               // folded FP32 output
-    // Digit output (Ucol non-null): the solution U of each quarter is rounded to the grid
-    // 2^(g-24) Z with 2^g > max|U| / 0.49 (g per quarter, in gexp[quarter]); Y receives the
-    // rounded U (exact in FP32) and Ucol / Urow the three signed base-256 digits of
-    // Z = c1 2^16 + c2 2^8 + c3, |c| <= 127, as bytes: Ucol[quarter][d][z'][y'] and the
-    // transposed Urow[quarter][d][y'][z'] (d = 0 for c1).
+    // Digit output (Ucol non-null, Y unused): the solution U of each quarter is rounded to the
+    // grid 2^(g-24) Z with 2^g > max|U| / 0.49 (g per quarter, in gexp[quarter]); Ucol / Urow
+    // receive the three signed base-256 digits of Z = c1 2^16 + c2 2^8 + c3, |c| <= 127, as
+    // bytes: Ucol[quarter][d][z'][y'] and the transposed Urow[quarter][d][y'][z'] (d = 0 for c1).
     unsigned char* Ucol;
     unsigned char* Urow;
     int* gexp;
@@ -80,7 +82,6 @@ struct ResidI8Args {
     const unsigned char* Ucol;          // digits of U (launch_lpq)
     const unsigned char* Urow;
     const int* gexp;
-    const float* U0;                    // U itself (launch_lpq's Y in digit mode, exact in FP32)
     float* R;                           // folded FP32 residual
     const i4v* Sdig[2];                 // S_e, S_o digits, layout of build_sdig
     const int* Sexp[2];                 // their row exponents, NH each

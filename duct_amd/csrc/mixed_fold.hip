@@ -111,8 +111,12 @@ struct UnfoldArgs {
     FieldGroup g;
     const float* src[kMaxFoldSources];
     int nsrc, NH, accumulate;
+    const unsigned char* udig;   // DIG: the first term in launch_lpq's digit form (Ucol, gexp)
+    const int* gexp;
+    int zero_edges;              // also write 0 to the boundary of sol and rhs (Dirichlet, helm3d.f90:48-60)
 };
 
+template <bool DIG>
 __global__ __launch_bounds__(256) void k_unfold_sum(UnfoldArgs a)
 {
     const FieldGroup& g = a.g;
@@ -124,14 +128,22 @@ __global__ __launch_bounds__(256) void k_unfold_sum(UnfoldArgs a)
     const int zq = (int)((idx % per) / he), yq = (int)(idx % he);
     int f, mode;
     plane_decode(g, fmp, f, mode);
+    const size_t NN = (size_t)NH * NH;
     d2v V[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         V[k] = d2v{0.0, 0.0};
 #pragma unroll
         for (int part = 0; part < 2; ++part) {
-            const size_t o = ((size_t)((2 * fmp + part) * 4 + k) * NH + zq) * NH + yq;
+            const size_t qi = (size_t)(2 * fmp + part) * 4 + k;
+            const size_t o = qi * NN + (size_t)zq * NH + yq;
             double s = 0.0;
+            if (DIG) {   // U = 2^(g-24) (c1 2^16 + c2 2^8 + c3), exact
+                const unsigned char* ud = a.udig + qi * kUDig * NN + (size_t)zq * NH + yq;
+                const int Z = (int)(signed char)ud[0] * 65536 + (int)(signed char)ud[NN] * 256 +
+                              (int)(signed char)ud[2 * NN];
+                s = ldexp((double)Z, a.gexp[qi] - 24);
+            }
             for (int t = 0; t < a.nsrc; ++t) s += (double)a.src[t][o];
             V[k][part] = s;
         }
@@ -153,22 +165,55 @@ __global__ __launch_bounds__(256) void k_unfold_sum(UnfoldArgs a)
         put(y0, z1, e1 + o1);
         if (y1 != y0) put(y1, z1, e1 - o1);
     }
+    if (a.zero_edges) {
+        // helm3d.f90:48-60: the boundary of rhs and sol set to zero; each thread takes the walls
+        // next to its points (y0, y1 cover 1..n, as z0, z1), the first thread the corners
+        d2v* in = const_cast<d2v*>(g.in[f]) + (size_t)mode * m * m;
+        auto zero = [&](size_t i) {
+            out[i] = d2v{0.0, 0.0};
+            in[i] = d2v{0.0, 0.0};
+        };
+        if (yq == 0) {
+            zero((size_t)z0 * m);
+            zero((m - 1) + (size_t)z0 * m);
+            zero((size_t)z1 * m);
+            zero((m - 1) + (size_t)z1 * m);
+        }
+        if (zq == 0) {
+            zero(y0);
+            zero(y1);
+            zero(y0 + (size_t)(m - 1) * m);
+            zero(y1 + (size_t)(m - 1) * m);
+        }
+        if (yq == 0 && zq == 0) {
+            zero(0);
+            zero(m - 1);
+            zero((size_t)(m - 1) * m);
+            zero((m - 1) + (size_t)(m - 1) * m);
+        }
+    }
 }
 
 hipError_t launch_unfold_sum(const FieldGroup& g, int NH, const float* const* src, int nsrc, int accumulate,
-                             hipStream_t st)
+                             hipStream_t st, const unsigned char* udig, const int* gexp, int zero_edges)
 {
-    if (nsrc < 1 || nsrc > kMaxFoldSources) return hipErrorInvalidValue;
+    if (nsrc < 0 || nsrc > kMaxFoldSources || (nsrc == 0 && !udig)) return hipErrorInvalidValue;
+    if (udig && !gexp) return hipErrorInvalidValue;
     UnfoldArgs a{};
     a.g = g;
     for (int t = 0; t < nsrc; ++t) a.src[t] = src[t];
     a.nsrc = nsrc;
     a.NH = NH;
     a.accumulate = accumulate;
+    a.udig = udig;
+    a.gexp = gexp;
+    a.zero_edges = zero_edges;
     const long he = (g.n + 1) / 2;
     const long total = (long)g.nf * g.nmodes * he * he;
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_unfold_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (udig) hipLaunchKernelGGL(k_unfold_sum<true>, grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_unfold_sum<false>, grid, dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -228,6 +273,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
                 ((uint4*)(a.Urow + qoff * kUDig))[e] = uint4{0u, 0u, 0u, 0u};
             }
             if (tid == 0) a.gexp[plane * 4 + q] = 0;
+            return;
         }
         for (int e = tid; e < NH * NH / 4; e += NTH) ((f4v*)Y)[e] = f4v{0.f, 0.f, 0.f, 0.f};
         return;
@@ -420,7 +466,6 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
             int ge = 0;
             if (gmx > 0.f && gmx < 3.0e38f) (void)frexpf(gmx / 0.49f, &ge);
             const float sc = ldexpf(1.f, 24 - ge);
-            const float isc = ldexpf(1.f, ge - 24);
             unsigned char* uc = a.Ucol + qoff * kUDig;
             unsigned char* ur = a.Urow + qoff * kUDig;
             // the 4 x 4 byte transpose within a lane quad (lanes l16 = 4 a .. 4 a + 3 hold columns
@@ -435,11 +480,9 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
                 for (int y = 0; y < TI; ++y) {
                     const int j = j0 + x * 16 + l4 * 4, i = i0 + y * 16 + l16;
                     unsigned w[kUDig] = {0u, 0u, 0u};
-                    f4v uq;
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         int Z = __float2int_rn(am[x][y][e] * sc);
-                        uq[e] = (float)Z * isc;   // |Z| < 2^23: exact
                         const int c3 = ((Z + 128) & 255) - 128;
                         Z = (Z - c3) >> 8;
                         const int c2 = ((Z + 128) & 255) - 128;
@@ -457,7 +500,6 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
                         const unsigned t2 = __builtin_amdgcn_perm(tx, t1, selB);
                         *(unsigned*)&ur[((size_t)d * NH + j + qv) * NH + (i - qv)] = t2;
                     }
-                    *(f4v*)&Y[(size_t)i * NH + j] = uq;
                 }
             if (tid == 0) a.gexp[plane * 4 + q] = ge;
     }

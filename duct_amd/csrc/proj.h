@@ -71,17 +71,20 @@ hipError_t launch_proj_fold(int NH, int mode, const ProjFoldArgs& a, hipStream_t
 
 // Padded folded size NH for interior n, or -1 when the folded kernels do not apply.
 int fold_size(int n);
-// Classify P's columns by parity and build the folded P^-1 (fwd) and P (bwd)
-// fragments, each (2 NH) x NH, and the eigenvalues per parity.  Returns -1 if
-// the eigenvectors do not split into ceil(n/2) even and floor(n/2) odd ones.
+// Classify P's columns by parity (the ceil(n/2) most even ones are even) and build the folded
+// P^-1 (fwd) and P (bwd) fragments, each (2 NH) x NH, and the eigenvalues per parity.  A column
+// that mixes the parities (an eigensolver's rotation of a nearly degenerate even/odd pair) is
+// projected onto its parity; the folded inverse is then recomputed and its eigenvalue taken
+// from the folded centrosymmetric part of op (the n x n interior operator, column-major;
+// required only then).  Returns -1 if the folded blocks cannot be built.
 // The folded matrices themselves: Af = [P_e^-1; P_o^-1], Ab = [P_e; P_o], each
 // (2 NH) x NH column-major (rows of parity p start at p NH).
 int fold_matrices(const double* P, const double* Pinv, const double* lam, int n, int NH, double* Af, double* Ab,
-                  double* lamf, int* pos);
+                  double* lamf, int* pos, const double* op);
 // pos (may be null) receives, for every eigenvector k, its index within its
 // parity (even: 0..h_e-1) or -1-index for odd ones.
 int build_fold_bases(const double* P, const double* Pinv, const double* lam, int n, int NH,
-                     double* fwd, double* bwd, double* lamf, int* pos);
+                     double* fwd, double* bwd, double* lamf, int* pos, const double* op);
 
 // Boundary treatment after pass 4 (one workgroup per complex plane).
 enum BoundaryMode : int {

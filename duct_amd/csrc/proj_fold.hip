@@ -401,11 +401,51 @@ int fold_size(int n)
     return -1;
 }
 
+namespace {
+// In-place LU with partial pivoting of an h x h column-major matrix; false if a pivot is 0.
+bool lu_factor(std::vector<double>& A, int h, std::vector<int>& piv)
+{
+    piv.resize(h);
+    for (int k = 0; k < h; ++k) {
+        int p = k;
+        for (int i = k + 1; i < h; ++i)
+            if (std::fabs(A[i + (size_t)k * h]) > std::fabs(A[p + (size_t)k * h])) p = i;
+        piv[k] = p;
+        if (A[p + (size_t)k * h] == 0.0) return false;
+        if (p != k)
+            for (int j = 0; j < h; ++j) std::swap(A[k + (size_t)j * h], A[p + (size_t)j * h]);
+        const double d = A[k + (size_t)k * h];
+        for (int i = k + 1; i < h; ++i) A[i + (size_t)k * h] /= d;
+        for (int j = k + 1; j < h; ++j) {
+            const double t = A[k + (size_t)j * h];
+            if (t != 0.0)
+                for (int i = k + 1; i < h; ++i) A[i + (size_t)j * h] -= A[i + (size_t)k * h] * t;
+        }
+    }
+    return true;
+}
+
+void lu_solve(const std::vector<double>& A, int h, const std::vector<int>& piv, double* b)
+{
+    for (int k = 0; k < h; ++k) std::swap(b[k], b[piv[k]]);
+    for (int k = 0; k < h; ++k)
+        for (int i = k + 1; i < h; ++i) b[i] -= A[i + (size_t)k * h] * b[k];
+    for (int k = h - 1; k >= 0; --k) {
+        b[k] /= A[k + (size_t)k * h];
+        for (int i = 0; i < k; ++i) b[i] -= A[i + (size_t)k * h] * b[k];
+    }
+}
+}  // namespace
+
 int fold_matrices(const double* P, const double* Pinv, const double* lam, int n, int NH, double* Af, double* Ab,
-                  double* lamf, int* pos)
+                  double* lamf, int* pos, const double* op)
 {
     const int he = (n + 1) / 2, ho = n / 2;
-    std::vector<int> ev, od;
+    // parity of every eigenvector: its even energy fraction; the he most even ones are the even
+    // ones.  A pair of eigenvalues of the two parities that nearly coincide may come back from
+    // the eigensolver as two mixtures of the pair's even and odd vectors: each then has a large
+    // share of both, and this split gives the pair one of each.
+    std::vector<double> fe(n), fmin(n);
     for (int k = 0; k < n; ++k) {
         double se = 0.0, so = 0.0;
         for (int i = 0; i < n; ++i) {
@@ -413,9 +453,23 @@ int fold_matrices(const double* P, const double* Pinv, const double* lam, int n,
             se += (a + b) * (a + b);
             so += (a - b) * (a - b);
         }
-        (se > so ? ev : od).push_back(k);
+        const double t = se + so;
+        if (!(t > 0.0)) return -1;
+        fe[k] = se / t;
+        fmin[k] = std::min(se, so) / t;
     }
-    if ((int)ev.size() != he || (int)od.size() != ho) return -1;
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return fe[x] > fe[y]; });
+    std::vector<char> even(n, 0);
+    for (int r = 0; r < he; ++r) even[order[r]] = 1;
+    std::vector<int> ev, od;
+    bool mixed = false;
+    for (int k = 0; k < n; ++k) {
+        (even[k] ? ev : od).push_back(k);
+        if ((even[k] && fe[k] < 0.5) || (!even[k] && fe[k] > 0.5)) mixed = true;   // a balanced split
+        if (fmin[k] > 1e-12) mixed = true;
+    }
     if (pos) {
         for (int r = 0; r < he; ++r) pos[ev[r]] = r;
         for (int r = 0; r < ho; ++r) pos[od[r]] = -1 - r;
@@ -441,15 +495,59 @@ int fold_matrices(const double* P, const double* Pinv, const double* lam, int n,
         }
         for (int r = h; r < NH; ++r) lamf[c * NH + r] = 0.0;
     }
+    if (!mixed) return 0;
+    // Mixed vectors: their parity parts span the pair's even and odd eigenvectors, so the
+    // projected columns Ab are still eigenvectors of the folded blocks, but the caller's P^-1
+    // rows no longer match them.  The folded inverse is then recomputed from Ab, and the
+    // eigenvalue of each mixed vector is its Rayleigh quotient on the folded block of the
+    // centrosymmetric part (op + J op J) / 2 of the interior operator.
+    if (!op) return -1;
+    for (int c = 0; c < 2; ++c) {
+        const std::vector<int>& ks = c ? od : ev;
+        const int h = c ? ho : he;
+        const double sg = c ? -1.0 : 1.0;
+        std::vector<double> V((size_t)h * h), Sb((size_t)h * h);
+        for (int r = 0; r < h; ++r)
+            for (int j = 0; j < h; ++j) V[j + (size_t)r * h] = Ab[(c * NH + j) + (size_t)r * R];
+        auto Sv = [&](int i, int j) { return 0.5 * (op[i + (size_t)j * n] + op[(n - 1 - i) + (size_t)(n - 1 - j) * n]); };
+        for (int j = 0; j < h; ++j)
+            for (int i = 0; i < h; ++i) {
+                const int jm = n - 1 - j;
+                Sb[i + (size_t)j * h] = (j == jm) ? Sv(i, j) : Sv(i, j) + sg * Sv(i, jm);
+            }
+        for (int r = 0; r < h; ++r) {
+            const int k = ks[r];
+            if (fmin[k] <= 1e-12 && ((c == 0) == (fe[k] > 0.5))) continue;
+            double num = 0.0, den = 0.0;
+            for (int i = 0; i < h; ++i) {
+                double sx = 0.0;
+                for (int j = 0; j < h; ++j) sx += Sb[i + (size_t)j * h] * V[j + (size_t)r * h];
+                num += V[i + (size_t)r * h] * sx;
+                den += V[i + (size_t)r * h] * V[i + (size_t)r * h];
+            }
+            if (!(den > 0.0)) return -1;
+            lamf[c * NH + r] = num / den;
+        }
+        std::vector<double> LU = V;
+        std::vector<int> piv;
+        if (!lu_factor(LU, h, piv)) return -1;
+        std::vector<double> col(h);
+        for (int j = 0; j < h; ++j) {       // column j of V^-1
+            std::fill(col.begin(), col.end(), 0.0);
+            col[j] = 1.0;
+            lu_solve(LU, h, piv, col.data());
+            for (int r = 0; r < h; ++r) Af[(c * NH + r) + (size_t)j * R] = col[r];
+        }
+    }
     return 0;
 }
 
 int build_fold_bases(const double* P, const double* Pinv, const double* lam, int n, int NH,
-                     double* fwd, double* bwd, double* lamf, int* pos)
+                     double* fwd, double* bwd, double* lamf, int* pos, const double* op)
 {
     const int R = 2 * NH;
     std::vector<double> Af((size_t)R * NH), Ab((size_t)R * NH);
-    if (fold_matrices(P, Pinv, lam, n, NH, Af.data(), Ab.data(), lamf, pos)) return -1;
+    if (fold_matrices(P, Pinv, lam, n, NH, Af.data(), Ab.data(), lamf, pos, op)) return -1;
     build_frag_rect(Af.data(), R, NH, R, fwd);
     build_frag_rect(Ab.data(), R, NH, R, bwd);
     return 0;

@@ -112,13 +112,15 @@ __device__ __forceinline__ void digit_products(const i4v (&uf)[2][kUDig][KC], co
 }  // namespace
 
 // Operands of one plane for one wave, loaded a plane ahead: the U digits of the first product
-// and, for the epilogue, f as its FP32 pair and U at the wave's output points.  Output tile
-// D[y][z] of 16 x 16 holds 4 consecutive y per lane (lane z%16 + 16 (y%16)/4, element y%4), so
-// every epilogue operand is one 16-byte load per tile from the column-major quarters.
+// and, for the epilogue, f as its FP32 pair and the digits of U at the wave's output points.
+// Output tile D[y][z] of 16 x 16 holds 4 consecutive y per lane (lane z%16 + 16 (y%16)/4,
+// element y%4), so f is one 16-byte load per tile from the column-major quarters and each digit
+// of U one 4-byte load.
 template <int KC>
 struct PlaneOps {
     i4v ub[2][kUDig][KC];   // B operand of the y-product: columns z of Ucol (k = y')
-    f4v fh[2][2], fl[2][2], u0[2][2];
+    f4v fh[2][2], fl[2][2];
+    unsigned uw[2][2][kUDig];
     int g;
 };
 
@@ -146,7 +148,9 @@ __device__ __forceinline__ void load_plane(const ResidI8Args& a, size_t qp, int 
             const size_t off = pt + (size_t)zt * 16 * NH + yt * 16;
             o.fh[yt][zt] = *(const f4v*)(a.Fhi + off);
             o.fl[yt][zt] = *(const f4v*)(a.Flo + off);
-            o.u0[yt][zt] = *(const f4v*)(a.U0 + off);
+            const size_t ud = qp * kUDig * NN + (size_t)(z0 + l16 + zt * 16) * NH + y0 + 4 * l4 + yt * 16;
+#pragma unroll
+            for (int d = 0; d < kUDig; ++d) o.uw[yt][zt][d] = *(const unsigned*)(a.Ucol + ud + d * NN);
         }
 }
 
@@ -249,7 +253,11 @@ __global__ __launch_bounds__(256, 1) void k_resid_i8(ResidI8Args a, long ncplx, 
                     const int y = y0 + yt * 16 + 4 * l4 + e, z = z0 + zt * 16 + l16;
                     const double lyv = ldexp(ly[yt][zt][e], ey[yt][e] + gq);
                     const double lz = ldexp(lzv[yt][zt][e], ez[zt] + gq);
-                    const double uv = (double)cur.u0[yt][zt][e];
+                    // U = 2^(g-24) (c1 2^16 + c2 2^8 + c3), the digits sign-extended from byte e
+                    const int c1 = (int)(cur.uw[yt][zt][0] << (24 - 8 * e)) >> 24;
+                    const int c2 = (int)(cur.uw[yt][zt][1] << (24 - 8 * e)) >> 24;
+                    const int c3 = (int)(cur.uw[yt][zt][2] << (24 - 8 * e)) >> 24;
+                    const double uv = ldexp((double)(c1 * 65536 + c2 * 256 + c3), gq - 24);
                     const double fv = (double)cur.fh[yt][zt][e] + (double)cur.fl[yt][zt][e];
                     double rv = fv - ((lyv + lz) - xlam * uv);
                     const bool in = y < hh[r] && z < hh[c];

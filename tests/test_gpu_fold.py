@@ -2,7 +2,7 @@
 
 Bar: relative error <= 1e-12 against the FP64 oracle, which runs the reference's
 unfolded products.  The folded basis is the parity projection of the
-reference's PD; on random data the two differ by ~1e-13 at m <= 257 (measured
+reference's PD (a nearly degenerate even/odd pair returned rotated included); on random data the two differ by ~1e-13 at m <= 257 (measured
 on the CPU with the oracle and a parity-projected basis), so 1e-12 is the
 plain FP64 bar, not a loosened one.  Sizes cover NH = 64, 128, 256 and odd and
 even n.
@@ -172,3 +172,64 @@ def test_fold_all_config2_batch(dev, oracle_mod):
                 assert e <= 1e-10
     finally:
         oracle_mod.use_blas(False)
+
+
+def _rotated_pair_basis(P, Pinv, lam, theta):
+    """P R and R^T P^-1 with R the rotation by theta of the even/odd eigenvector pair whose
+    eigenvalues are closest (relative): what an eigensolver may return for a nearly degenerate
+    pair.  Returns the new (P, P^-1), the pair and its relative gap."""
+    J = P[::-1]
+    se, so = ((P + J) ** 2).sum(0), ((P - J) ** 2).sum(0)
+    fe = se / (se + so)
+    ev, od = np.where(fe > 0.5)[0], np.where(fe <= 0.5)[0]
+    L1, L2 = lam[ev][:, None], lam[od][None, :]
+    gap = np.abs(L1 - L2) / np.maximum(np.abs(L1), np.abs(L2))
+    i, j = np.unravel_index(np.argmin(gap), gap.shape)
+    a, b = ev[i], od[j]
+    c, s = np.cos(theta), np.sin(theta)
+    P2, Pi2 = P.copy(), Pinv.copy()
+    P2[:, a], P2[:, b] = c * P[:, a] + s * P[:, b], -s * P[:, a] + c * P[:, b]
+    Pi2[a], Pi2[b] = c * Pinv[a] + s * Pinv[b], -s * Pinv[a] + c * Pinv[b]
+    return P2, Pi2, (a, b), float(gap[i, j])
+
+
+@pytest.mark.parametrize("bound,theta", [("d", np.pi / 4), ("d", 0.5), ("n", np.pi / 4)])
+def test_fold_with_rotated_near_degenerate_pair(dev, oracle_mod, bound, theta):
+    """A basis whose nearly degenerate even/odd eigenvector pair comes back rotated (mixed
+    parities, as another eigensolver may return it; the pair's relative gap is ~2e-5 at
+    m = 257) still folds: each mixed column is projected onto its parity, the folded inverse is
+    recomputed and the pair's eigenvalues come from the folded operator.  The folded solve then
+    agrees with the oracle as closely as with the unrotated basis (FP64 folded passes; for 'd'
+    also the int8-residual mixed solve)."""
+    m, M = 257, 3
+    ob = ensure(dev, oracle_mod, m)
+    s = dev.solver._state["setup"]
+    b = s.dirichlet if bound == "d" else s.neumann
+    rng = np.random.default_rng(31 + (bound == "n"))
+    f = rng.standard_normal((M, m, m)) + 1j * rng.standard_normal((M, m, m))
+    if bound == "d":
+        f[:, 0, :] = f[:, -1, :] = f[:, :, 0] = f[:, :, -1] = 0
+    alam = 1000.0 if bound == "d" else 2.5
+    mode = 1 if bound == "d" else 2
+    ref, _ = oracle_mod.helm3d(ob, f, bound, alam)
+    base = relerr(solve(dev, f, bound, alam, mode), ref)
+    P2, Pi2, pair, gap = _rotated_pair_basis(b.P, b.Pinv, b.lam, theta)
+    print("rotated pair %s (relative gap %.1e), theta %.3f" % (pair, gap, theta), flush=True)
+    dev.solver.upload_basis(bound, m, P2, Pi2, b.lam, s.DC, b.AM1, b.iofzero, b.wanted)
+    try:
+        assert dev.fold_size(bound) > 0
+        got = solve(dev, f, bound, alam, mode)
+        e = relerr(got, ref)
+        print("%s folded: unrotated basis %.2e, rotated %.2e" % (bound, base, e), flush=True)
+        assert e <= max(2.0 * base, 1e-13)
+        if bound == "d":
+            import torch
+            r = torch.from_numpy(np.ascontiguousarray(f)).cuda()
+            u = torch.zeros_like(r)
+            dev.helm3d_mixed_batch([r], [u], "d", [alam], scheme0="fold16q", scheme1="fold16q", niter=1)
+            torch.cuda.synchronize()
+            em = relerr(u.cpu().numpy(), ref)
+            print("d fold16q with the rotated basis: %.2e" % em, flush=True)
+            assert em <= 1e-10
+    finally:
+        dev.solver.upload_basis(bound, m, b.P, b.Pinv, b.lam, s.DC, b.AM1, b.iofzero, b.wanted)
