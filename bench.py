@@ -426,6 +426,8 @@ def main():
         # the stage's algorithmic work (8 n^3 per real plane, four unfolded products) over the step time;
         # the folded and refined paths execute other amounts, which the roofline counts
         "algorithmic_tflops": 4 * g / (res[chosen]["ms_per_step"] * 1e-3) / 1e12,
+        # the same over the FP64 MFMA peak: above 1 for the folded paths, which execute half the FLOPs
+        "algorithmic_frac_of_f64_peak": 4 * g / (res[chosen]["ms_per_step"] * 1e-3) / PEAK["f64"],
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[unit_peak] / 1e12,
                      "unit": "TOP/s" if unit_peak == "i8" else "TFLOP/s", "frac": achieved / PEAK[unit_peak],
                      "traffic": TRAFFIC.get((dom, m, M, F)), "kernel": kernel_name, "peak_kind": unit_peak,
