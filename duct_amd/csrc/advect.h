@@ -6,8 +6,8 @@
 
 namespace duct {
 
-constexpr int kXprodMaxM = 64;    // complex x-modes kept (2/3 rule: mx1 + 1)
-constexpr int kXprodMaxX = 192;   // physical x points (mgalx)
+constexpr int kXprodMaxM = 64;    // the dense kernels' largest complex x-mode count (2/3 rule: mx1 + 1)
+constexpr int kXprodMaxX = 192;   // and physical x point count (mgalx); rocFFT past them
 
 struct XprodArgs {
     const d2v* in[3];      // u, v, w: complex(8) (m, m, M)
@@ -47,5 +47,28 @@ void build_colloc_matrices(int m, std::vector<double>& d1, std::vector<double>& 
 // products of k_cline_fold on the even and odd parts of their input).
 void build_colloc_folded(int m, std::vector<double>& fold1, std::vector<double>& fold2);
 void build_xdft(int M, int X, std::vector<double>& einv, std::vector<double>& fwd);
+
+// Sizes beyond the dense x-DFT kernels: batched rocFFT real transforms (capi.hip) between these
+// element-wise stages (xfft.hip).  H = X/2 + 1 Hermitian modes of an X-point real line.
+struct XpadArgs {
+    const d2v* in[3];      // mode-major (m, m, M) fields, or one line-major fou[line][M]
+    d2v* out;              // mode-major: [f][h][line]; line-major: [line][h]; h < H
+    long lines;
+    int nf, M, H, X;
+};
+// mode h < M copied, h >= M zero, Im of the zero mode (and of the Nyquist mode) zero
+hipError_t launch_xpad(const XpadArgs& a, bool line_major, hipStream_t st);
+// p[0..5] (each n doubles) = uu, vv, ww, uv, uw, vw
+hipError_t launch_xmul6(const double* u, const double* v, const double* w, double* p, long n, hipStream_t st);
+struct XpackArgs {
+    const d2v* in;         // mode-major: [f][h][line] (nf fields); line-major: [line][h]
+    d2v* out[6];           // mode-major: (m, m, M) fields; line-major: fou[line][M]
+    long lines;
+    int nf, M, H, X, line_major;
+};
+// out = in / X for the modes h < M
+hipError_t launch_xpack(const XpackArgs& a, hipStream_t st);
+// phys(X+2, lines): the two values past X of every line set to 0
+hipError_t launch_xzero_tail(double* phys, int X, long lines, hipStream_t st);
 
 }  // namespace duct

@@ -16,6 +16,8 @@
 #include <utility>
 #include <vector>
 
+#include <rocfft/rocfft.h>
+
 #include "../../include/duct_hip.h"
 #include "advect.h"
 #include "aux.h"
@@ -146,6 +148,23 @@ struct Lib {
     int xdft_M = 0, xdft_X = 0;
     d2v* prods = nullptr;                 // the six product fields of the advection term
     size_t prods_bytes = 0;
+    // x-transforms past the dense kernels' sizes: rocFFT plans (kept per layout), their work
+    // buffer and execution info, the Hermitian and the physical lines
+    struct XfftPlan {
+        int forward;
+        size_t X, batch, is, id, os, od;
+        rocfft_plan plan;
+        size_t work;
+    };
+    std::vector<XfftPlan> xplans;
+    bool rocfft_on = false;
+    rocfft_execution_info xinfo = nullptr;
+    void* xwork = nullptr;
+    size_t xwork_bytes = 0;
+    d2v* xw = nullptr;
+    size_t xw_bytes = 0;
+    double* xp = nullptr;
+    size_t xp_bytes = 0;
     hipEvent_t ev_switch = nullptr;       // orders a newly set stream after the previous one
     bool noalloc = false;                 // inside duct_helm3d_batch_noalloc_dev_: nothing may allocate
     void* scratch_ptr[kNoAllocDevices] = {};  // per device: the address of the static scratch array
@@ -1146,19 +1165,114 @@ int cline_run(int mm, int M, std::initializer_list<ClineOp> ops)
 // k_xprod, then the nine derivative terms in the order of operators.F:91-272 per output
 // (adv1: uu_x, uv_y, uw_z; adv2: vv_y, uv_x, vw_z; adv3: ww_z, uw_x, vw_y); the caller has
 // checked the sizes.
+// The dense x-DFT kernels (k_xprod, k_fourx) cover M % 16 == 0, M <= 64, X % 16 == 0, X <= 192;
+// other sizes go through batched rocFFT real transforms.
+bool dense_x(int M, int X)
+{
+    return M % 16 == 0 && M <= kXprodMaxM && X % 16 == 0 && X <= kXprodMaxX;
+}
+
+// One batched real transform of X-point lines on the library's stream: forward (r2c: real in,
+// X/2+1 Hermitian out) or inverse (c2r; its input may be overwritten).  Strides and distances in
+// elements (doubles for the real side, complex values for the Hermitian side).  Plans are kept.
+int xfft_run(bool forward, void* in, void* out, size_t X, size_t batch, size_t is, size_t id, size_t os, size_t od)
+{
+    if (!L.rocfft_on) {
+        if (rocfft_setup() != rocfft_status_success) return fail(ERR_HIP, "rocfft_setup failed");
+        L.rocfft_on = true;
+    }
+    Lib::XfftPlan* pl = nullptr;
+    for (auto& p : L.xplans)
+        if (p.forward == (int)forward && p.X == X && p.batch == batch && p.is == is && p.id == id && p.os == os &&
+            p.od == od)
+            pl = &p;
+    if (!pl) {
+        rocfft_plan_description desc = nullptr;
+        if (rocfft_plan_description_create(&desc) != rocfft_status_success)
+            return fail(ERR_HIP, "rocfft_plan_description_create failed");
+        const rocfft_array_type ti = forward ? rocfft_array_type_real : rocfft_array_type_hermitian_interleaved;
+        const rocfft_array_type to = forward ? rocfft_array_type_hermitian_interleaved : rocfft_array_type_real;
+        rocfft_status rs = rocfft_plan_description_set_data_layout(desc, ti, to, nullptr, nullptr, 1, &is, id, 1, &os, od);
+        rocfft_plan plan = nullptr;
+        if (rs == rocfft_status_success)
+            rs = rocfft_plan_create(&plan, rocfft_placement_notinplace,
+                                    forward ? rocfft_transform_type_real_forward : rocfft_transform_type_real_inverse,
+                                    rocfft_precision_double, 1, &X, batch, desc);
+        (void)rocfft_plan_description_destroy(desc);
+        if (rs != rocfft_status_success) return fail(ERR_HIP, "rocfft_plan_create failed (%d)", (int)rs);
+        size_t work = 0;
+        if (rocfft_plan_get_work_buffer_size(plan, &work) != rocfft_status_success) {
+            (void)rocfft_plan_destroy(plan);
+            return fail(ERR_HIP, "rocfft_plan_get_work_buffer_size failed");
+        }
+        L.xplans.push_back(Lib::XfftPlan{(int)forward, X, batch, is, id, os, od, plan, work});
+        pl = &L.xplans.back();
+    }
+    if (!L.xinfo && rocfft_execution_info_create(&L.xinfo) != rocfft_status_success)
+        return fail(ERR_HIP, "rocfft_execution_info_create failed");
+    if (rocfft_execution_info_set_stream(L.xinfo, L.stream) != rocfft_status_success)
+        return fail(ERR_HIP, "rocfft_execution_info_set_stream failed");
+    if (pl->work) {
+        int st = ensure_buf(L.xwork, L.xwork_bytes, pl->work);
+        if (st) return st;
+        if (rocfft_execution_info_set_work_buffer(L.xinfo, L.xwork, pl->work) != rocfft_status_success)
+            return fail(ERR_HIP, "rocfft_execution_info_set_work_buffer failed");
+    }
+    void* ib[1] = {in};
+    void* ob[1] = {out};
+    if (rocfft_execute(pl->plan, ib, ob, L.xinfo) != rocfft_status_success) return fail(ERR_HIP, "rocfft_execute failed");
+    HIPCHK(hipGetLastError());
+    return OK;
+}
+
+// The six products of the advection term (k_xprod's output, prods (m, m, M) each) through rocFFT:
+// fill (modes M..X/2 zero), c2r of u, v, w, the products in physical space, r2c, pack with 1/X.
+int xprod_fft(const d2v* const in[3], d2v* prods, int mm, int M, int X)
+{
+    const long lines = (long)mm * mm;
+    const int H = X / 2 + 1;
+    int st;
+    if ((st = ensure_buf(L.xw, L.xw_bytes, (size_t)6 * H * lines * sizeof(d2v)))) return st;
+    if ((st = ensure_buf(L.xp, L.xp_bytes, (size_t)9 * X * lines * sizeof(double)))) return st;
+    Timed t(PK_XPROD);
+    XpadArgs pa{};
+    for (int f = 0; f < 3; ++f) pa.in[f] = in[f];
+    pa.out = L.xw;
+    pa.lines = lines; pa.nf = 3; pa.M = M; pa.H = H; pa.X = X;
+    HIPCHK(launch_xpad(pa, false, L.stream));
+    double* phys = L.xp;                          // u, v, w: [line][x]
+    double* prodp = L.xp + (size_t)3 * X * lines; // uu, vv, ww, uv, uw, vw: [line][x]
+    for (int f = 0; f < 3; ++f)
+        if ((st = xfft_run(false, L.xw + (size_t)f * H * lines, phys + (size_t)f * X * lines, X, lines, lines, 1, 1, X)))
+            return st;
+    HIPCHK(launch_xmul6(phys, phys + (size_t)X * lines, phys + (size_t)2 * X * lines, prodp, (long)X * lines,
+                        L.stream));
+    for (int p = 0; p < 6; ++p)
+        if ((st = xfft_run(true, prodp + (size_t)p * X * lines, L.xw + (size_t)p * H * lines, X, lines, 1, X, lines, 1)))
+            return st;
+    XpackArgs ka{};
+    ka.in = L.xw;
+    for (int p = 0; p < 6; ++p) ka.out[p] = prods + (size_t)p * M * lines;
+    ka.lines = lines; ka.nf = 6; ka.M = M; ka.H = H; ka.X = X; ka.line_major = 0;
+    HIPCHK(launch_xpack(ka, L.stream));
+    return OK;
+}
+
 int advection_run(const d2v* const in[3], d2v* const out[3], int mm, int M, int X, double alp, double aspect)
 {
-    int st = ensure_xdft(M, X);
-    if (st) return st;
+    int st;
     const size_t field = (size_t)M * mm * mm;
     if ((st = ensure_buf(L.prods, L.prods_bytes, 6 * field * sizeof(d2v)))) return st;
-    XprodArgs xa{};
-    for (int i = 0; i < 3; ++i) xa.in[i] = in[i];
-    for (int p = 0; p < 6; ++p) xa.out[p] = L.prods + p * field;
-    xa.Einv = L.xdft[0];
-    xa.Fwd = L.xdft[1];
-    xa.m = mm; xa.M = M; xa.X = X;
-    {
+    if (!dense_x(M, X)) {
+        if ((st = xprod_fft(in, L.prods, mm, M, X))) return st;
+    } else {
+        if ((st = ensure_xdft(M, X))) return st;
+        XprodArgs xa{};
+        for (int i = 0; i < 3; ++i) xa.in[i] = in[i];
+        for (int p = 0; p < 6; ++p) xa.out[p] = L.prods + p * field;
+        xa.Einv = L.xdft[0];
+        xa.Fwd = L.xdft[1];
+        xa.m = mm; xa.M = M; xa.X = X;
         Timed t(PK_XPROD);
         HIPCHK(launch_xprod(xa, L.stream));
     }
@@ -1247,6 +1361,16 @@ int duct_shutdown_(void)
     L.xdft_M = L.xdft_X = 0;
     free_ptr(L.prods);
     L.prods_bytes = 0;
+    for (auto& xp : L.xplans) (void)rocfft_plan_destroy(xp.plan);
+    L.xplans.clear();
+    if (L.xinfo) (void)rocfft_execution_info_destroy(L.xinfo);
+    L.xinfo = nullptr;
+    if (L.rocfft_on) (void)rocfft_cleanup();
+    L.rocfft_on = false;
+    free_ptr(L.xwork);
+    free_ptr(L.xw);
+    free_ptr(L.xp);
+    L.xwork_bytes = L.xw_bytes = L.xp_bytes = 0;
     free_ptr(L.mfold);
     L.mfold_bytes = 0;
     free_ptr(L.mdig);
@@ -1929,18 +2053,40 @@ int duct_fourx_dev_(const duct_devptr_t* fou, const duct_devptr_t* phys, const i
     if (!*fou || !*phys) return fail(ERR_ARG, "null device pointer");
     if (*my < 1 || *nz < 1) return fail(ERR_ARG, "my=%d, nz=%d", *my, *nz);
     const int M = *nmodes, X = *mgalx;
-    if (M % 16 || M > kXprodMaxM || X % 16 || X > kXprodMaxX || 2 * M > X)
-        return fail(ERR_UNSUPPORTED, "fourx needs nmodes %% 16 == 0, nmodes <= %d, mgalx %% 16 == 0, "
-                    "2 nmodes <= mgalx <= %d (got %d, %d)", kXprodMaxM, kXprodMaxX, M, X);
+    if (M < 1 || X < 2 || X % 2 || 2 * M > X)
+        return fail(ERR_ARG, "fourx needs nmodes >= 1 and an even mgalx >= 2 nmodes (got %d, %d)", M, X);
     int st = ensure_init();
     if (st) return st;
+    const long lines = (long)*my * *nz;
+    if (!dense_x(M, X)) {   // batched rocFFT along x: fou(0:M-1, lines) <-> phys(X+2, lines)
+        const int H = X / 2 + 1;
+        if ((st = ensure_buf(L.xw, L.xw_bytes, (size_t)H * lines * sizeof(d2v)))) return st;
+        Timed t(PK_XPROD);
+        if (*iopt >= 0) {
+            XpadArgs pa{};
+            pa.in[0] = (const d2v*)*fou;
+            pa.out = L.xw;
+            pa.lines = lines; pa.nf = 1; pa.M = M; pa.H = H; pa.X = X;
+            HIPCHK(launch_xpad(pa, true, L.stream));
+            if ((st = xfft_run(false, L.xw, (void*)*phys, X, lines, 1, H, 1, X + 2))) return st;
+            HIPCHK(launch_xzero_tail((double*)*phys, X, lines, L.stream));
+        } else {
+            if ((st = xfft_run(true, (void*)*phys, L.xw, X, lines, 1, X + 2, 1, H))) return st;
+            XpackArgs ka{};
+            ka.in = L.xw;
+            ka.out[0] = (d2v*)*fou;
+            ka.lines = lines; ka.nf = 1; ka.M = M; ka.H = H; ka.X = X; ka.line_major = 1;
+            HIPCHK(launch_xpack(ka, L.stream));
+        }
+        return OK;
+    }
     if ((st = ensure_xdft(M, X))) return st;
     FourxArgs a{};
     a.fou = (d2v*)*fou;
     a.phys = (double*)*phys;
     a.Einv = L.xdft[0];
     a.Fwd = L.xdft[1];
-    a.lines = (long)*my * *nz;
+    a.lines = lines;
     a.M = M; a.X = X; a.inverse = *iopt >= 0;
     Timed t(PK_XPROD);
     HIPCHK(launch_fourx(a, L.stream));
@@ -1963,9 +2109,8 @@ int duct_advection_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const du
     if (st) return st;
     if ((st = check_common(*m, *nmodes))) return st;
     const int M = *nmodes, X = *mgalx, mm = *m;
-    if (M % 16 || M > kXprodMaxM || X % 16 || X > kXprodMaxX || X < 3 * M)
-        return fail(ERR_UNSUPPORTED, "advection needs nmodes %% 16 == 0, nmodes <= %d, mgalx %% 16 == 0, "
-                    "3 nmodes <= mgalx <= %d (got %d, %d)", kXprodMaxM, kXprodMaxX, M, X);
+    if (X % 2 || X < 3 * M)
+        return fail(ERR_ARG, "advection needs an even mgalx >= 3 nmodes (the 3/2 rule; got %d, %d)", M, X);
     if (!(*aspect > 0.0)) return fail(ERR_ARG, "aspect must be positive");
     if (!L.dcfrag[0] || !L.dfold[0] || L.dc_m != mm)
         return fail(ERR_NOT_SETUP, "no collocation matrix for m=%d (duct_setup_ builds it)", mm);
@@ -2100,9 +2245,8 @@ int duct_rk3_substep_dev_(const duct_devptr_t* u, const duct_devptr_t* v, const 
     if (st) return st;
     const int mm = *m, M = *nmodes, X = *mgalx;
     if ((st = check_common(mm, M))) return st;
-    if (M % 16 || M > kXprodMaxM || X % 16 || X > kXprodMaxX || X < 3 * M)
-        return fail(ERR_UNSUPPORTED, "the substep's advection needs nmodes %% 16 == 0, nmodes <= %d, mgalx %% 16 == 0, "
-                    "3 nmodes <= mgalx <= %d (got %d, %d)", kXprodMaxM, kXprodMaxX, M, X);
+    if (X % 2 || X < 3 * M)
+        return fail(ERR_ARG, "the substep's advection needs an even mgalx >= 3 nmodes (got %d, %d)", M, X);
     if (!L.dfold[0] || !L.dfold[1] || L.dc_m != mm)
         return fail(ERR_NOT_SETUP, "no collocation matrices for m=%d (duct_setup_ builds them)", mm);
     for (int b = 0; b < 2; ++b)

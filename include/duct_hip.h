@@ -187,8 +187,9 @@ int duct_colloc_deriv_dev_(const duct_devptr_t *in, const duct_devptr_t *out, co
  * iopt >= 0: fou -> phys, modes above nmodes-1 zero (fill), FFTW c2r sum
  * (Im of mode 0 ignored), the two padding values written as 0; iopt < 0:
  * phys -> fou, the first nmodes modes of the r2c sum times 1/mgalx
- * (pack_fftw).  Dense x-DFTs on the FP64 matrix cores; nmodes % 16 == 0,
- * nmodes <= 64, mgalx % 16 == 0, 2 nmodes <= mgalx <= 192.                  */
+ * (pack_fftw).  mgalx even, 2 nmodes <= mgalx.  Dense x-DFTs on the FP64
+ * matrix cores for nmodes % 16 == 0, nmodes <= 64, mgalx % 16 == 0,
+ * mgalx <= 192; batched rocFFT real transforms otherwise.                   */
 int duct_fourx_dev_(const duct_devptr_t *fou, const duct_devptr_t *phys, const int *my, const int *nz,
                     const int *nmodes, const int *mgalx, const int *iopt);
 
@@ -242,8 +243,8 @@ int duct_helm3d_temp_dev_(const duct_devptr_t *rhs, const duct_devptr_t *sol, co
  * hold advection3d of the previous substep's velocity on entry (read when krk > 1)
  * and of this substep's input velocity on exit.  dpdx is the caller's driving
  * gradient (get_dpdx, operators.F:2478-2522: -fnu dpval(aspect)); 0 for none.
- * The advection needs nmodes % 16 == 0, nmodes <= 64, mgalx % 16 == 0 and
- * 3 nmodes <= mgalx <= 192.  Both bounds must be set up for m.                    */
+ * The advection needs an even mgalx >= 3 nmodes (dense x-DFTs up to 64 modes and
+ * 192 points, rocFFT past that).  Both bounds must be set up for m.               */
 int duct_rk3_substep_dev_(const duct_devptr_t *u, const duct_devptr_t *v, const duct_devptr_t *w,
                           const duct_devptr_t *p, const duct_devptr_t *adv1, const duct_devptr_t *adv2,
                           const duct_devptr_t *adv3, const int *m, const int *nmodes, const int *mgalx,
@@ -282,8 +283,8 @@ int duct_derivx_dev_(const duct_devptr_t *in, const duct_devptr_t *out, const in
  * with the products formed on mgalx physical x points by the 3/2 rule (fourx,
  * mult, pack_fftw: src/fourier.F:1-115, src/mult.F) and d/dx = i alp k,
  * d/dy = DC, d/dz = DC / aspect (the collocation form of transyz + derivyz,
- * src/chebyshev.f90).  Needs duct_setup_ for m; nmodes % 16 == 0,
- * nmodes <= 64, mgalx % 16 == 0, 3 nmodes <= mgalx <= 192.  The adv fields must
+ * src/chebyshev.f90).  Needs duct_setup_ for m and an even mgalx >= 3 nmodes
+ * (dense x-DFTs up to 64 modes and 192 points, rocFFT past that).  The adv fields must
  * not alias u, v, w.  The reference's slip-velocity, CFL and statistics side
  * outputs of advection3d are not computed.                                  */
 int duct_advection_dev_(const duct_devptr_t *u, const duct_devptr_t *v, const duct_devptr_t *w,

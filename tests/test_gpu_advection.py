@@ -2,7 +2,8 @@
 step (192 x-points, 64 modes, 129^2 CGL points), against the oracle,
 which follows the reference's own route (FFTW-convention x transforms, REDFT00
 Chebyshev transforms and the derivyz recurrence, oracle/spectral.py) where the
-GPU uses dense x-DFTs on the matrix cores and collocation products.
+GPU uses dense x-DFTs on the matrix cores (up to 64 modes and 192 points, batched
+rocFFT transforms past that) and collocation products.
 
 Bars: advection 1e-12 relative (max norm per component); the full step 1e-10
 (it contains the k=0 pressure-Poisson solve, ~1e-11 conditioning at this size).
@@ -89,7 +90,7 @@ def test_advection_aspect_and_errors(dev):
     for c in range(3):
         assert relerr(got[c], ref[c]) <= 1e-12
     with pytest.raises(DuctError):
-        run_adv(dev, [f[:8] for f in fields], 48, 1.0)             # nmodes % 16 != 0
+        run_adv(dev, fields, 47, 1.0)                               # odd mgalx
     with pytest.raises(DuctError):
         run_adv(dev, fields, 40, 1.0)                               # mgalx < 3 nmodes
     import torch
@@ -97,6 +98,24 @@ def test_advection_aspect_and_errors(dev):
     u = [cuda(f) for f in fields]
     with pytest.raises(DuctError):
         step.advection(u[0], u[1], u[2], u[0], torch.zeros_like(u[0]), torch.zeros_like(u[0]), mgalx=48)
+
+
+# Past the dense x-DFT kernels (M % 16 != 0, M > 64 or mgalx > 192): batched rocFFT real transforms
+# along x with the fill, the products in physical space and the pack, then the same derivative terms.
+# The last case is the full solve workload's grid: 257^2 x 256 modes, mgalx 768.
+@pytest.mark.parametrize("m,M,mgalx,alp", [(17, 8, 24, 1.1), (33, 20, 64, 1.0), (65, 96, 288, 0.9),
+                                          (257, 256, 768, 1.0)])
+def test_advection_large_modes_matches_oracle(dev, m, M, mgalx, alp):
+    from oracle import spectral
+    ensure(dev, m)
+    fields = random_velocity(m, M, 7 * m + M)
+    print("advection m=%d M=%d mgalx=%d (rocFFT x-transforms)" % (m, M, mgalx), flush=True)
+    got = run_adv(dev, fields, mgalx, alp)
+    ref = spectral.advection3d(*fields, mgalx=mgalx, alp=alp)
+    for c in range(3):
+        e = relerr(got[c], ref[c])
+        print("m=%d M=%d mgalx=%d adv%d rel.err %.2e" % (m, M, mgalx, c + 1, e), flush=True)
+        assert e <= 1e-12
 
 
 def test_config3_step(dev, oracle_mod):
@@ -199,9 +218,11 @@ def test_transyz_and_derivyz_match_oracle(dev, m):
     assert np.max(np.abs(p.cpu().numpy() - want)) <= 1e-12 * np.abs(DC).sum(1).max() * np.abs(f).max()
 
 
-@pytest.mark.parametrize("my,nz,M,mgalx", [(9, 5, 16, 48), (129, 7, 64, 192)])
+@pytest.mark.parametrize("my,nz,M,mgalx", [(9, 5, 16, 48), (129, 7, 64, 192), (17, 9, 100, 300), (257, 33, 256, 768),
+                                          (12, 3, 5, 10)])
 def test_fourx_matches_oracle(dev, my, nz, M, mgalx):
-    """duct_fourx_dev_ against numpy's FFT restatement of fourx (fill, c2r, r2c, pack_fftw)."""
+    """duct_fourx_dev_ against numpy's FFT restatement of fourx (fill, c2r, r2c, pack_fftw); the
+    dense kernel up to 64 modes and 192 points (multiples of 16), rocFFT past that."""
     import torch
     from duct_amd import step
     from oracle import spectral as S

@@ -65,7 +65,7 @@ def run_gpu(fields, adv, krks, fnu, dt, mgalx, alp, dpdx):
     return out
 
 
-@pytest.mark.parametrize("m,M,mgalx", [(33, 16, 48), (40, 32, 96), (200, 16, 48)])
+@pytest.mark.parametrize("m,M,mgalx", [(33, 16, 48), (40, 32, 96), (200, 16, 48), (33, 24, 72)])
 def test_rk3_step_matches_oracle(dev, oracle_mod, m, M, mgalx):
     """A full time step (krk = 1, 2, 3) from the same state, every substep compared."""
     from oracle import rk3 as R
