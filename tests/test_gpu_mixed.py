@@ -55,8 +55,12 @@ SCHEMES = {
          ("bf16x6", "bf16x6", 1)],
     129: [("bf16x3", "bf16x6", 2), ("bf16x3", "bf16x3", 2), ("f32", "f32", 1), ("bf16x6", "bf16x6", 1),
           ("fold16", "fold16", 1), ("fold16q", "fold16q", 1)],
+    # even interiors: n = 128 (both halves 64) and n = 198 (halves of 99, a ragged last group of
+    # four points in the fold and unfold)
+    130: [("fold16q", "fold16q", 1)],
+    200: [("fold16", "fold16", 1), ("fold16q", "fold16q", 1)],
 }
-CASES = [(m, M, s0, s1, it) for m, M in ((33, 16), (129, 8)) for s0, s1, it in SCHEMES[m]]
+CASES = [(m, M, s0, s1, it) for m, M in ((33, 16), (129, 8), (130, 4), (200, 4)) for s0, s1, it in SCHEMES[m]]
 
 
 @pytest.mark.parametrize("m,M,s0,s1,niter", CASES)
