@@ -18,6 +18,10 @@ struct XprodArgs {
 };
 // M % 16 == 0, M <= 64, X % 16 == 0, X <= 192.
 hipError_t launch_xprod(const XprodArgs& a, hipStream_t st);
+// The same products for X = 192 and M <= 64 with the x-transforms as 64-point FFTs across the
+// lanes of DPP rows (xlane.hip; Einv and Fwd unused); tab = build_xlane_table on the device.
+hipError_t launch_xprod_lane(const XprodArgs& a, const d2v* tab, hipStream_t st);
+void build_xlane_table(std::vector<double>& tab);
 
 struct FourxArgs {
     d2v* fou;              // fou(0:M-1, lines) complex, mode fastest

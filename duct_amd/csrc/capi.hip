@@ -145,6 +145,7 @@ struct Lib {
     int dc_m = 0, dc_np = 0;
     int fold = 0;                         // 1: Dirichlet groups use the folded passes, 2: Neumann too
     double* xdft[2] = {nullptr, nullptr}; // advection x-DFTs (inverse, forward), fragment layout
+    d2v* xlane = nullptr;                  // twiddles of the 192-point lane FFTs (build_xlane_table)
     int xdft_M = 0, xdft_X = 0;
     d2v* prods = nullptr;                 // the six product fields of the advection term
     size_t prods_bytes = 0;
@@ -1263,7 +1264,20 @@ int advection_run(const d2v* const in[3], d2v* const out[3], int mm, int M, int 
     int st;
     const size_t field = (size_t)M * mm * mm;
     if ((st = ensure_buf(L.prods, L.prods_bytes, 6 * field * sizeof(d2v)))) return st;
-    if (!dense_x(M, X)) {
+    if (X == 192 && M <= 64) {   // the x-transforms as 64-point FFTs across DPP rows
+        if (!L.xlane) {
+            std::vector<double> tab;
+            build_xlane_table(tab);
+            HIPCHK(hipMalloc((void**)&L.xlane, tab.size() * sizeof(double)));
+            HIPCHK(hipMemcpy(L.xlane, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        XprodArgs xa{};
+        for (int i = 0; i < 3; ++i) xa.in[i] = in[i];
+        for (int p = 0; p < 6; ++p) xa.out[p] = L.prods + p * field;
+        xa.m = mm; xa.M = M; xa.X = X;
+        Timed t(PK_XPROD);
+        HIPCHK(launch_xprod_lane(xa, L.xlane, L.stream));
+    } else if (!dense_x(M, X)) {
         if ((st = xprod_fft(in, L.prods, mm, M, X))) return st;
     } else {
         if ((st = ensure_xdft(M, X))) return st;
@@ -1356,6 +1370,7 @@ int duct_shutdown_(void)
     L.tfield_bytes = 0;
     free_ptr(L.bscr);
     L.bscr_bytes = 0;
+    free_ptr(L.xlane);
     free_ptr(L.xdft[0]);
     free_ptr(L.xdft[1]);
     L.xdft_M = L.xdft_X = 0;

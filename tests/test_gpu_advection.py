@@ -65,7 +65,11 @@ def run_adv(dev, fields, mgalx, alp, aspect=1.0):
 
 # m = 257 and 288 run the folded derivative products with three row tiles per wave (the largest
 # instantiation; 288 is the largest m of the derivative kernels)
+# mgalx = 192 with up to 64 modes runs the lane-FFT product kernel (xlane.hip): 129^2 x 64 (the
+# pseudo-spectral workload), fewer modes than the transform length (48, 17), and line counts that
+# leave the last workgroup ragged (17^2, 33^2)
 @pytest.mark.parametrize("m,M,mgalx,alp", [(17, 16, 48, 1.3), (33, 32, 96, 1.0), (40, 16, 64, 0.7), (129, 64, 192, 1.0),
+                                          (33, 48, 192, 1.1), (17, 64, 192, 0.9), (65, 17, 192, 1.0),
                                           (257, 16, 48, 1.0), (288, 16, 48, 0.8)])
 def test_advection_matches_oracle(dev, m, M, mgalx, alp):
     from oracle import spectral
