@@ -61,8 +61,9 @@ PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "f16": 2.5e15,   # MI35
         "i8": 5.0e15}                                                   # FLOP/s (int8: OP/s)
 # HBM bytes per launch of a kernel class at a workload (m, nmodes, nfields), from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters over a plain 1 GiB copy
-# (profiles/k_projf_f64_fold_257x256x4_pmc.md: fetch counted 0.5 per byte, write 1.0).
-TRAFFIC = {("fold", 257, 256, 4): 1116.4e6}
+# (profiles/k_projf_f64_fold_257x256x4_pmc.md, profiles/k_lpq_fold16q_257x256x4_traffic.md: fetch
+# counted 0.5 per byte, write 1.0).
+TRAFFIC = {("fold", 257, 256, 4): 1116.4e6, ("lp_p2p", 257, 256, 4): 493.0e6}
 TOL = 1e-10        # the metric's tolerance (refined and folded paths)
 TOL_FP64 = 1e-12   # the FP64 bar (plain FP64 paths)
 
@@ -178,9 +179,11 @@ def c3_bench(steps, warmup):
     lines, X, M = cfg.m * cfg.m, 192, cfg.nmodes
     xp = out["advection3d"]["kernels"].get("xprod")
     if xp:
-        flop = 36.0 * lines * X * M          # dense x-DFTs: 3 inverse (Q = 2M), 6 forward (Re, Im)
-        out["advection3d"]["xprod_dense_dft_tflops"] = flop / (xp["avg_us"] * 1e-6) / 1e12
-        out["advection3d"]["xprod_hbm_gbs"] = 9 * lines * M * 16 / (xp["avg_us"] * 1e-6) / 1e9
+        # algorithmic bytes of the product kernel: 3 fields of M complex modes in and 6 products out
+        # per (y, z) line, against the HBM peak (the x-transforms run as FFTs on the vector units)
+        gbs = 9 * lines * M * 16 / (xp["avg_us"] * 1e-6) / 1e9
+        out["advection3d"]["xprod_hbm_gbs"] = gbs
+        out["advection3d"]["xprod_hbm_frac"] = gbs * 1e9 / 8.0e12
     print(json.dumps({"workload": "pseudo-spectral step, %d^2 CGL x %d modes, mgalx 192" % (cfg.m, M),
                       "steps": steps, **out}), flush=True)
     duct_amd.shutdown()
