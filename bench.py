@@ -293,7 +293,19 @@ def main():
         if kind.startswith("proj_"):
             return g, "f64"
         if kind.startswith("fold_"):
-            return gf, "f64"
+            # folded fields: the Dirichlet ones (fp64_fold_d) or all; the fused middle passes skip
+            # the singular Neumann planes (alambda = 0, mode 0), which the separate passes take
+            # one complex plane (two real planes) at a time
+            nd = sum(1 for c in bounds if c == "d")
+            pf = (nd if path == "fp64_fold_d" else F) * (2 * M - 1)
+            nsing = 0 if path == "fp64_fold_d" else sum(1 for c, al in zip(bounds, alams) if c == "n" and al == 0.0)
+            per = 2.0 * (he * he + ho * ho) * n
+            fused = kt_kinds.get(path, set())
+            if kind == "fold_mid":                 # k_projq: passes 2 and 3 in one launch
+                return 2.0 * per * (pf - nsing), "f64"
+            if kind in ("fold_p2p_div", "fold_p2p") and "fold_mid" in fused:
+                return per * 2 * nsing, "f64"
+            return per * pf, "f64"
         if path in ("mixed_fold", "mixed_fold_q"):
             if kind == "lp_p2p":                   # k_lpq: 4 passes x 3 fp16 products, two solves
                 return 2 * 4 * 3 * gf, "f16"
@@ -310,6 +322,7 @@ def main():
         return 0.0, None
 
     res = {}
+    kt_kinds = {}                                  # the kernel classes each path launched
     # --path X times only X (a profiled run then holds only X's kernels)
     for path in (paths if args.path == "auto" else (args.path,)):
         for _ in range(args.warmup):
@@ -332,6 +345,7 @@ def main():
             dt_ev = (time.perf_counter() - t0) / args.steps
         if args.events_only:
             dt = dt_ev
+        kt_kinds[path] = {k for k in kt.ms if kt.count[k]}
         classes = {}
         for k in kt.ms:
             if not kt.count[k]:
@@ -373,6 +387,8 @@ def main():
     for k, v in ks.items():
         key = "proj" if k.startswith("proj_") else "lp" if k.startswith("lp_") and chosen not in (
             "mixed_fold", "mixed_fold_q") else "fold" if k.startswith("fold_") else k
+        if key == "fold" and k in ("fold_p2p_div", "fold_p2p") and "fold_mid" in ks:
+            key = "fold_singular"                  # the singular planes' separate passes
         groups.setdefault(key, []).append(k)
     dom = max(groups, key=lambda gk: sum(ks[k]["ms_per_step"] for k in groups[gk]))
     dom_ms = sum(ks[k]["ms_per_step"] for k in groups[dom])

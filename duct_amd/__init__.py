@@ -13,5 +13,5 @@ Layout
 from . import _lib  # noqa: F401
 from .solver import (  # noqa: F401
     KernelTimer, fold_size, helm3d, helm3d_batch, helm3d_host, helm3d_mixed_batch, init, int2d3d, set_fold,
-    set_overlap, set_stream, setup, shutdown, solvediag, synchronize, upload_basis,
+    set_fused_quarters, set_overlap, set_stream, setup, shutdown, solvediag, synchronize, upload_basis,
 )

@@ -50,6 +50,7 @@ SIGNATURES = {
     "duct_mat_invdiag_dev_": (_i, [_pi, _pi, _pd, _psz, _psz, _psz]),
     "duct_set_fold_": (_i, [_pi]),
     "duct_set_overlap_": (_i, [_pi]),
+    "duct_set_fused_quarters_": (_i, [_pi]),
     "duct_fold_size_": (_i, [_pc, _pi, _sz]),
     "duct_prof_enable_": (_i, [_pi]),
     "duct_h5_read_header_": (_i, [_pc, _pi, _pd, _sz]),
@@ -64,7 +65,7 @@ STATUS = {0: "OK", 1: "DUCT_ERR_ARG", 2: "DUCT_ERR_NOT_SETUP", 3: "DUCT_ERR_HIP"
 
 PROF_CLASSES = ["proj_c2p", "proj_p2p_div", "gauge", "proj_p2p", "proj_p2c", "boundary", "bc_correct",
                 "transpose", "invdiag", "lp_c2p", "lp_p2p_div", "lp_p2p", "lp_p2c", "resid", "deriv", "adjoint",
-                "fold_c2p", "fold_p2p_div", "fold_p2p", "fold_p2c", "xprod", "tridiag"]
+                "fold_c2p", "fold_p2p_div", "fold_p2p", "fold_p2c", "xprod", "tridiag", "fold_mid"]
 
 
 class DuctError(RuntimeError):

@@ -114,7 +114,8 @@ struct ProfRec {
 enum ProfKind { PK_C2P = 0, PK_P2P_DIV = 1, PK_GAUGE = 2, PK_P2P = 3, PK_P2C = 4, PK_BOUNDARY = 5, PK_BCCORR = 6,
                 PK_TRANSPOSE = 7, PK_INVDIAG = 8, PK_LP_C2P = 9, PK_LP_P2P_DIV = 10, PK_LP_P2P = 11,
                 PK_LP_P2C = 12, PK_RESID = 13, PK_DERIV = 14, PK_ADJ = 15, PK_F_C2P = 16,
-                PK_F_P2P_DIV = 17, PK_F_P2P = 18, PK_F_P2C = 19, PK_XPROD = 20, PK_TRIDIAG = 21 };
+                PK_F_P2P_DIV = 17, PK_F_P2P = 18, PK_F_P2C = 19, PK_XPROD = 20, PK_TRIDIAG = 21,
+                PK_F_MID = 22 };
 
 struct Lib {
     bool init = false;
@@ -144,6 +145,7 @@ struct Lib {
     size_t bscr_bytes = 0;
     int dc_m = 0, dc_np = 0;
     int fold = 0;                         // 1: Dirichlet groups use the folded passes, 2: Neumann too
+    int fuse_quarters = 1;                // folded passes 2 and 3 on LDS-resident quarters where built
     double* xdft[2] = {nullptr, nullptr}; // advection x-DFTs (inverse, forward), fragment layout
     d2v* xlane = nullptr;                  // twiddles of the 192-point lane FFTs (build_xlane_table)
     int xdft_M = 0, xdft_X = 0;
@@ -359,35 +361,25 @@ int bound_index(char c)
     return -1;
 }
 
-// The four projection passes, the gauge and the boundary for one group of
-// fields sharing a basis.  bmode is a BoundaryMode.
-// The four passes in the even/odd folded basis (Dirichlet), then the boundary.
-int run_group_fold(const FieldGroup& g, int bidx, int bmode)
+// Folded passes 2 and 3 as two planar launches, with the Neumann gauge between them on the
+// singular planes: X (pass-1 output) -> G (undivided) in W -> X (pass-3 output).
+int run_fold_middle(const FieldGroup& g, const DevBasis& B, double* X, double* W)
 {
-    DevBasis& B = L.basis[bidx];
     const int NH = B.NH;
-    const long cols = (long)g.nf * g.nmodes * NH;
-    const long planes = (long)g.nf * g.nmodes * 2;
-    int st = ensure_planar((size_t)planes * 4 * NH * NH * sizeof(double));
-    if (st) return st;
+    const bool neu = &B == &L.basis[1];
     ProjFoldArgs a{};
     a.g = g;
     a.lam = B.flam;
-    a.ncols = cols;
-    a.gauge = bidx == 1 ? B.fio : -1;
-    {
-        Timed t(PK_F_C2P);
-        a.Afrag = B.ffwd; a.X = nullptr; a.Y = L.plan[0];
-        HIPCHK(launch_proj_fold(NH, PROJ_C2P, a, L.stream));
-    }
+    a.ncols = (long)g.nf * g.nmodes * NH;
+    a.gauge = neu ? B.fio : -1;
     {   // pass 2 undivided: the divide is applied by pass 3 as it loads G
         Timed t(PK_F_P2P_DIV);
-        a.X = L.plan[0]; a.Y = L.plan[1];
+        a.Afrag = B.ffwd; a.X = X; a.Y = W;
         HIPCHK(launch_proj_fold(NH, PROJ_P2P, a, L.stream));
     }
-    if (bidx == 1) {  // singular planes: |xlambda| <= 1e-10 (solvediag.f90:151)
+    if (neu) {  // singular planes: |xlambda| <= 1e-10 (solvediag.f90:151)
         GaugeFoldArgs ga{};
-        ga.G = L.plan[1]; ga.prow = B.fprow; ga.pdiag = B.fpdiag; ga.wanted = B.wanted; ga.lam = B.flam;
+        ga.G = W; ga.prow = B.fprow; ga.pdiag = B.fpdiag; ga.wanted = B.wanted; ga.lam = B.flam;
         ga.io = B.fio; ga.NH = NH; ga.g = g;
         GaugeList gl{};
         for (int f = 0; f < g.nf; ++f)
@@ -410,12 +402,62 @@ int run_group_fold(const FieldGroup& g, int bidx, int bmode)
     }
     {
         Timed t(PK_F_P2P);
-        a.Afrag = B.fbwd; a.X = L.plan[1]; a.Y = L.plan[0];
+        a.Afrag = B.fbwd; a.X = W; a.Y = X;
         HIPCHK(launch_proj_fold(NH, PROJ_P2P_LDIV, a, L.stream));
+    }
+    return OK;
+}
+
+// The four passes in the even/odd folded basis, then the boundary.  bmode is a BoundaryMode.
+int run_group_fold(const FieldGroup& g, int bidx, int bmode)
+{
+    DevBasis& B = L.basis[bidx];
+    const int NH = B.NH;
+    const long cols = (long)g.nf * g.nmodes * NH;
+    const long planes = (long)g.nf * g.nmodes * 2;
+    int st = ensure_planar((size_t)planes * 4 * NH * NH * sizeof(double));
+    if (st) return st;
+    ProjFoldArgs a{};
+    a.g = g;
+    a.lam = B.flam;
+    a.ncols = cols;
+    a.gauge = bidx == 1 ? B.fio : -1;
+    {
+        Timed t(PK_F_C2P);
+        a.Afrag = B.ffwd; a.X = nullptr; a.Y = L.plan[0];
+        HIPCHK(launch_proj_fold(NH, PROJ_C2P, a, L.stream));
+    }
+    if (L.fuse_quarters && proj_quarter_supported(NH)) {
+        {   // passes 2 and 3 on LDS-resident quarters, in place on plan[0]
+            Timed t(PK_F_MID);
+            ProjQuarterArgs q{};
+            q.fwd = B.ffwd; q.bwd = B.fbwd; q.lam = B.flam; q.X = L.plan[0];
+            q.nplanes = planes; q.gauge = a.gauge; q.g = g;
+            HIPCHK(launch_proj_quarter(NH, q, L.stream));
+        }
+        if (a.gauge >= 0) {   // the singular planes it skips: the separate passes and the gauge, per plane
+            for (int f = 0; f < g.nf; ++f)
+                for (int k = 0; k < g.nmodes; ++k) {
+                    if (!(std::fabs(xlambda_of(g.alambda[f], g.alp, g.mode0 + k)) <= 1.0e-10)) continue;
+                    FieldGroup g1 = g;
+                    g1.nf = 1;
+                    g1.nmodes = 1;
+                    g1.mode0 = g.mode0 + k;
+                    g1.alambda[0] = g.alambda[f];
+                    g1.in[0] = g.in[f] + (size_t)k * g.m * g.m;
+                    g1.out[0] = g.out[f] + (size_t)k * g.m * g.m;
+                    const size_t off = (size_t)(f * g.nmodes + k) * 2 * 4 * NH * NH;
+                    int st1 = run_fold_middle(g1, B, L.plan[0] + off, L.plan[1] + off);
+                    if (st1) return st1;
+                }
+        }
+    } else {
+        int st1 = run_fold_middle(g, B, L.plan[0], L.plan[1]);
+        if (st1) return st1;
     }
     {
         Timed t(PK_F_P2C);
-        a.X = L.plan[0]; a.Y = nullptr;
+        a.Afrag = B.fbwd; a.X = L.plan[0]; a.Y = nullptr;
         HIPCHK(launch_proj_fold(NH, PROJ_P2C, a, L.stream));
     }
     {
@@ -1407,6 +1449,15 @@ int duct_set_overlap_(const int* on)
     int st = ensure_init();
     if (st) return st;
     L.overlap = *on ? 1 : 0;
+    return OK;
+}
+
+int duct_set_fused_quarters_(const int* on)
+{
+    if (!on) return fail(ERR_ARG, "null argument to duct_set_fused_quarters_");
+    int st = ensure_init();
+    if (st) return st;
+    L.fuse_quarters = *on ? 1 : 0;
     return OK;
 }
 

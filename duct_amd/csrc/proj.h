@@ -69,6 +69,22 @@ struct ProjFoldArgs {
 };
 hipError_t launch_proj_fold(int NH, int mode, const ProjFoldArgs& a, hipStream_t st);
 
+// Passes 2 and 3 of the folded solve (undivided product, eigenvalue divide, product) on
+// LDS-resident quarters, one workgroup per quarter, in place on the planar buffer.  Same
+// results as PROJ_P2P followed by PROJ_P2P_LDIV.  Singular Neumann planes (gauge >= 0 and
+// |xlambda| <= 1e-10) and the Im planes of mode 0 are left untouched.
+struct ProjQuarterArgs {
+    const double* fwd;    // folded P^-1 [P_e^-1; P_o^-1], fragment layout
+    const double* bwd;    // folded P [P_e; P_o], fragment layout
+    const double* lam;    // [lam_e | lam_o], NH each, zero padded
+    double* X;            // planar quarters: pass-1 output in, pass-3 output out
+    long nplanes;         // real planes: nf * nmodes * 2
+    int gauge;            // index of the Neumann null mode among the even eigenvectors, -1 if none
+    FieldGroup g;
+};
+bool proj_quarter_supported(int NH);
+hipError_t launch_proj_quarter(int NH, const ProjQuarterArgs& a, hipStream_t st);
+
 // Padded folded size NH for interior n, or -1 when the folded kernels do not apply.
 int fold_size(int n);
 // Classify P's columns by parity (the ceil(n/2) most even ones are even) and build the folded

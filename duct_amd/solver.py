@@ -159,6 +159,12 @@ def set_overlap(on: bool = True) -> None:
     _lib.call("duct_set_overlap_", _lib.byref_int(1 if on else 0))
 
 
+def set_fused_quarters(on: bool = True) -> None:
+    """Folded passes 2 and 3 of each 128 x 128 quarter in one kernel with the quarter in LDS
+    (duct_set_fused_quarters_; on by default); off: two planar launches.  Same results to the bit."""
+    _lib.call("duct_set_fused_quarters_", _lib.byref_int(1 if on else 0))
+
+
 def fold_size(bound: str) -> int:
     """Folded size NH of the basis set up for bound (0: the basis does not fold)."""
     v = ctypes.c_int(0)

@@ -340,6 +340,16 @@ int duct_set_fold_(const int *mode);
  * one field per helm3d call and has no such switch.                         */
 int duct_set_overlap_(const int *on);
 
+/* Execution of the folded solve's middle passes (duct_set_fold_ 1 and 2).
+ * on != 0 (default): where the folded size is 128 (m = 131 ... 259), the
+ * undivided second product, the eigenvalue divide and the third product of
+ * each 128 x 128 quarter run in one kernel, with the quarter held in LDS; the
+ * singular Neumann planes, whose gauge sums over all four quarters, keep the
+ * separate passes.  on = 0: two planar launches with the gauge between them.
+ * The results are the same to the bit.  The reference has no such switch
+ * (its products are cuBLAS calls, solvediag.f90:51-80, :140-219).          */
+int duct_set_fused_quarters_(const int *on);
+
 /* Folded size NH (padded ceil(n/2)) of the basis set up for bound, 0 when it
  * does not fold (n < 65, or eigenvectors without a clean parity split).     */
 int duct_fold_size_(const char *bound, int *NH, size_t bound_len);
@@ -394,7 +404,8 @@ int duct_prof_enable_(const int *on);
  * 1 proj_p2p_div, 2 gauge, 3 proj_p2p, 4 proj_p2c, 5 boundary, 6 bc_correct,
  * 7 transpose, 8 invdiag, 9 lp_c2p, 10 lp_p2p_div, 11 lp_p2p, 12 lp_p2c,
  * 13 resid, 14 deriv, 15 adjoint (Neumann boundary adjoint), 16 fold_c2p,
- * 17 fold_p2p_div, 18 fold_p2p, 19 fold_p2c, 20 xprod (advection products), 21 tridiag (Shen). */
+ * 17 fold_p2p_div, 18 fold_p2p, 19 fold_p2c, 20 xprod (advection products), 21 tridiag (Shen),
+ * 22 fold_mid (folded passes 2 and 3 on LDS-resident quarters).                                */
 int duct_prof_read_(const int *nclass, double *ms, int *count);
 
 #ifdef __cplusplus

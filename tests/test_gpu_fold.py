@@ -233,3 +233,42 @@ def test_fold_with_rotated_near_degenerate_pair(dev, oracle_mod, bound, theta):
             assert em <= 1e-10
     finally:
         dev.solver.upload_basis(bound, m, b.P, b.Pinv, b.lam, s.DC, b.AM1, b.iofzero, b.wanted)
+
+
+@pytest.mark.parametrize("m,M,ib", [(131, 3, 0), (200, 3, 0), (257, 3, 0), (257, 2, 5), (258, 2, 0)])
+def test_fused_quarters_bit_identical(dev, oracle_mod, m, M, ib):
+    """Folded passes 2 and 3 on LDS-resident quarters (duct_set_fused_quarters_, the default at
+    folded size 128) against the two planar launches they replace: the same MFMA sequence and
+    divide, so the u, v, w, p batch (Neumann Poisson gauge planes included at ib = 0, skipped by
+    the fused kernel) must agree to the bit; padded quarters (m = 131: 65 of 128 rows), ragged
+    even n (m = 200, 258) and a mode offset are covered.  One mode is also checked against the
+    oracle."""
+    import torch
+    from duct_amd import workloads
+    ensure(dev, oracle_mod, m)
+    assert dev.fold_size("d") == 128 and dev.fold_size("n") == 128
+    rng = np.random.default_rng(11 * m + ib)
+    fields = [rng.standard_normal((M, m, m)) + 1j * rng.standard_normal((M, m, m)) for _ in range(4)]
+    alams = [1000.0, 1000.0, 1000.0, 0.0]
+
+    def run(fused):
+        rhs = [torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in fields]
+        sol = [torch.zeros_like(r) for r in rhs]
+        dev.set_fold(2)
+        dev.set_fused_quarters(fused)
+        try:
+            dev.helm3d_batch(rhs, sol, workloads.BOUNDS, alams, ib=ib)
+            torch.cuda.synchronize()
+        finally:
+            dev.set_fold(0)
+            dev.set_fused_quarters(True)
+        return [s.cpu().numpy() for s in sol]
+
+    a, b = run(True), run(False)
+    for f in range(4):
+        assert np.array_equal(a[f], b[f]), (f, float(np.max(np.abs(a[f] - b[f]))))
+    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(m))
+    ref, _ = oracle_mod.helm3d(ob, fields[0][1:2], "d", 1000.0, ib=ib + 1)
+    e = relerr(a[0][1:2], ref)
+    print("m=%d ib=%d fused quarters: bit-identical, Dirichlet mode %d rel.err %.2e" % (m, ib, ib + 1, e), flush=True)
+    assert e <= TOL
