@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
                         const int hh = rp ? (n >> 1) : he;
                         const int hc = ld_cp ? (n >> 1) : he;
                         if (ld_l < hc && k < hh && !(ld_gauge && rp == 0 && k == a.gauge))
-                            v = v / (lam_s[rp * NH + k] + ld_lamc);
+                            v = div_eig(v, lam_s[rp * NH + k] + ld_lamc);
                     }
                     Bs[buf][rp][v_dst * LDB + kk0 + e] = v;
                 }

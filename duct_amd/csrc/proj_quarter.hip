@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kQThreads, 1) void k_projq(ProjQuarterArgs a)
                     for (int r = 0; r < 4; ++r) {
                         const int c = wc * 64 + nt * 16 + l16, i = wr * 32 + mt * 16 + 4 * r + l4;
                         const double v = acc[mt][nt][r];
-                        const double qv = v / (lc[nt] + li[mt][r]);
+                        const double qv = div_eig(v, lc[nt] + li[mt][r]);
                         out[mt][nt][r] = (i < hh[rp] && c < hh[cp]) ? qv : v;
                     }
 #pragma unroll
