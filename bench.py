@@ -61,9 +61,9 @@ PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "f16": 2.5e15,   # MI35
         "i8": 5.0e15}                                                   # FLOP/s (int8: OP/s)
 # HBM bytes per launch of a kernel class at a workload (m, nmodes, nfields), from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters over a plain 1 GiB copy
-# (profiles/k_projf_f64_fold_257x256x4_pmc.md, profiles/k_lpq_fold16q_257x256x4_traffic.md: fetch
-# counted 0.5 per byte, write 1.0).
-TRAFFIC = {("fold", 257, 256, 4): 1116.4e6, ("lp_p2p", 257, 256, 4): 493.0e6}
+# (profiles/k_projf_k_projq_f64_fold_257x256x4_pmc.md, profiles/k_lpq_fold16q_257x256x4_traffic.md:
+# fetch counted 0.5 per byte, write 1.0).
+TRAFFIC = {("fold", 257, 256, 4): 1133.4e6, ("fold_mid", 257, 256, 4): 1117.6e6, ("lp_p2p", 257, 256, 4): 493.0e6}
 TOL = 1e-10        # the metric's tolerance (refined and folded paths)
 TOL_FP64 = 1e-12   # the FP64 bar (plain FP64 paths)
 
@@ -314,6 +314,8 @@ def main():
                     return 2.0 * gf, "f64"
                 return 2 * 18 * gf, "i8"           # two products x 18 int8 digit products
             return 0.0, None                       # fold / unfold: bandwidth-bound
+        if path not in schemes:
+            return 0.0, None                       # gauge, boundary: not matrix-core work
         a, b, it = schemes[path]
         if kind.startswith("lp_"):
             return g * (nprod[a] + it * nprod[b]), ("f32" if a == "f32" else "bf16")
@@ -387,24 +389,39 @@ def main():
     for k, v in ks.items():
         key = "proj" if k.startswith("proj_") else "lp" if k.startswith("lp_") and chosen not in (
             "mixed_fold", "mixed_fold_q") else "fold" if k.startswith("fold_") else k
-        if key == "fold" and k in ("fold_p2p_div", "fold_p2p") and "fold_mid" in ks:
+        if k == "fold_mid":
+            key = "fold_mid"                       # k_projq, its own kernel
+        elif key == "fold" and k in ("fold_p2p_div", "fold_p2p") and "fold_mid" in ks:
             key = "fold_singular"                  # the singular planes' separate passes
         groups.setdefault(key, []).append(k)
+    kernel_names = {"proj": "k_proj (FP64 MFMA projection pass)",
+                    "fold": "k_projf (FP64 MFMA projection pass, even/odd folded basis)",
+                    "fold_mid": "k_projq (FP64 MFMA folded passes 2 and 3 on LDS-resident quarters)",
+                    "lp": "k_proj_lp (low-precision MFMA projection pass)",
+                    "lp_p2p": "k_lpq (fused per-quarter fp16 hi/lo solve)",
+                    "resid": {"mixed_fold": "k_resid_fold (FP64 MFMA folded residual)",
+                              "mixed_fold_q": "k_resid_i8 (int8 digit products, FP64-accurate residual)"}.get(
+                                  chosen, "k_resid (FP64 MFMA residual)")}
+
+    def roofline_of(gk):
+        ms = sum(ks[k]["ms_per_step"] for k in groups[gk])
+        launches = sum(ks[k]["launches_per_step"] for k in groups[gk])
+        work = sum(work_per_step(k, chosen)[0] for k in groups[gk])
+        unit = work_per_step(groups[gk][0], chosen)[1] or "f64"
+        avg_s = ms / 1e3 / launches if launches else float("nan")
+        per_launch = work / launches if launches else 0.0
+        ach = per_launch / avg_s if launches else 0.0
+        return {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[unit] / 1e12,
+                "unit": "TOP/s" if unit == "i8" else "TFLOP/s", "frac": ach / PEAK[unit],
+                "traffic": TRAFFIC.get((gk, m, M, F)), "kernel": kernel_names.get(gk, gk), "peak_kind": unit,
+                "ops": "executed", "ops_per_launch": per_launch, "avg_launch_us": avg_s * 1e6,
+                "ms_per_step": ms, "launches_per_step": launches}
+
+    mfma_groups = [gk for gk in groups if work_per_step(groups[gk][0], chosen)[0] > 0]
     dom = max(groups, key=lambda gk: sum(ks[k]["ms_per_step"] for k in groups[gk]))
-    dom_ms = sum(ks[k]["ms_per_step"] for k in groups[dom])
-    dom_launches = sum(ks[k]["launches_per_step"] for k in groups[dom])
-    dom_work = sum(work_per_step(k, chosen)[0] for k in groups[dom])
-    unit_peak = work_per_step(groups[dom][0], chosen)[1] or "f64"
-    avg_launch_s = dom_ms / 1e3 / dom_launches if dom_launches else float("nan")
-    work_per_launch = dom_work / dom_launches if dom_launches else 0.0
-    achieved = work_per_launch / avg_launch_s if dom_launches else 0.0
-    kernel_name = {"proj": "k_proj (FP64 MFMA projection pass)",
-                   "fold": "k_projf (FP64 MFMA projection pass, even/odd folded basis)",
-                   "lp": "k_proj_lp (low-precision MFMA projection pass)",
-                   "lp_p2p": "k_lpq (fused per-quarter fp16 hi/lo solve)",
-                   "resid": {"mixed_fold": "k_resid_fold (FP64 MFMA folded residual)",
-                             "mixed_fold_q": "k_resid_i8 (int8 digit products, FP64-accurate residual)"}.get(
-                                 chosen, "k_resid (FP64 MFMA residual)")}.get(dom, dom)
+    roof = roofline_of(dom)
+    # every matrix-core kernel class of the step, longest first
+    roof_all = sorted((roofline_of(gk) for gk in mfma_groups), key=lambda r: -r["ms_per_step"])
 
     fp64_best = fastest([p for p in res if res[p]["tier"] == "fp64" and res[p].get("meets_bar", refs is None)])
     mixed_best = fastest([p for p in res if res[p]["tier"] == "mixed" and res[p].get("meets_bar", False)])
@@ -447,10 +464,8 @@ def main():
         "algorithmic_tflops": 4 * g / (res[chosen]["ms_per_step"] * 1e-3) / 1e12,
         # the same over the FP64 MFMA peak: above 1 for the folded paths, which execute half the FLOPs
         "algorithmic_frac_of_f64_peak": 4 * g / (res[chosen]["ms_per_step"] * 1e-3) / PEAK["f64"],
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[unit_peak] / 1e12,
-                     "unit": "TOP/s" if unit_peak == "i8" else "TFLOP/s", "frac": achieved / PEAK[unit_peak],
-                     "traffic": TRAFFIC.get((dom, m, M, F)), "kernel": kernel_name, "peak_kind": unit_peak,
-                     "ops": "executed", "ops_per_launch": work_per_launch, "avg_launch_us": avg_launch_s * 1e6},
+        "roofline": roof,
+        "roofline_kernels": roof_all,
         "paths": res,
         "fp64_mfma_frac": None,
         "cpu_baseline": cpu,

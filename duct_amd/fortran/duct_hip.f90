@@ -10,7 +10,7 @@ module duct_hip
   private
   public :: duct_init, duct_shutdown, duct_setup, duct_helm3d, duct_helm3d_dev, &
             duct_helm3d_batch_dev, duct_int2d3d, duct_solvediag, duct_check, duct_set_fold, &
-            duct_set_overlap, &
+            duct_set_overlap, duct_set_fused_quarters, &
             duct_helm3d_mixed_batch_dev, duct_helm3d_adjoint_batch_dev, duct_colloc_deriv_dev, &
             duct_derivx_dev, duct_advection_dev, duct_transyz_dev, duct_derivyz_dev, duct_fourx_dev, &
             duct_rk3_substep_dev, duct_setup_temp, duct_helm3d_temp_dev, &
@@ -107,6 +107,12 @@ module duct_hip
 
     ! Neumann group of a batched solve on a second stream (1, default) or in order (0)
     integer(c_int) function duct_set_overlap(on) bind(C, name="duct_set_overlap_")
+      import :: c_int
+      integer(c_int), intent(in) :: on
+    end function
+
+    ! folded passes 2 and 3 on LDS-resident quarters (1, default) or as two planar launches (0)
+    integer(c_int) function duct_set_fused_quarters(on) bind(C, name="duct_set_fused_quarters_")
       import :: c_int
       integer(c_int), intent(in) :: on
     end function
