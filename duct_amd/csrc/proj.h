@@ -71,18 +71,15 @@ hipError_t launch_proj_fold(int NH, int mode, const ProjFoldArgs& a, hipStream_t
 
 // The eigenvalue divide of the folded solve, a / d for a finite d of moderate magnitude (an
 // eigenvalue sum, |d| well inside the normal range; no scaling for overflow, underflow or special
-// values): v_rcp_f64 refined by two Newton steps, then the quotient corrected by its residual
-// (Markstein): 8 FP64 operations against 11 for the fully scaled IEEE sequence.  On gfx950 it
-// returned the IEEE quotient in all of 4.2e6 random pairs (|a| in 1e-6 .. 1e9, d in -2e9 .. -1e-6).
-// The folded passes that divide (k_projf's divide on load, k_projq) share it, so their results
-// agree to the bit.
+// values): v_rcp_f64 refined by one Newton step, then the quotient corrected by its residual
+// (Markstein): 6 FP64 operations against 11 for the fully scaled IEEE sequence.  On gfx950 it
+// returned the IEEE quotient in all of 4.2e6 random pairs (|a| in 1e-6 .. 1e9, d in -2e9 .. -1e-6),
+// as did the same with two Newton steps.  The folded passes that divide (k_projf's divide on load,
+// k_projq) share it, so their results agree to the bit.
 __device__ inline double div_eig(double a, double d)
 {
     double r = __builtin_amdgcn_rcp(d);
-    double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
+    r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
     const double q = a * r;
     return __builtin_fma(r, __builtin_fma(-d, q, a), q);
 }
