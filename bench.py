@@ -10,15 +10,16 @@ Paths timed on the same inputs:
                group, the reference's unfolded products (agrees with the oracle to ~1e-15)
   fp64_fold_d  the Dirichlet fields in the even/odd folded eigenbasis (half the FLOPs;
                ~1e-13 of the oracle), the Neumann field unfolded
-  fp64_fold    every field folded, the Neumann k=0 gauge included (~5e-11 of the
-               oracle on the Poisson planes, their own conditioning)
+  fp64_fold    every field folded except the Poisson modes with xlambda < 64 (the
+               singular k=0 plane among them), which take the unfolded passes
+               (~2e-13 of the oracle)
   mixed        the refined form of --mixed (default bf16x6,bf16x6,1): low-precision
                passes (FP32 or split-bf16 MFMA), FP64 residual, correction(s)
                accumulated into the FP64 solution
   mixed_f32    the same with FP32-MFMA passes (variant A), --f32-iter refinements
   mixed_fold   the refined form in the folded basis (scheme fold16): fused per-quarter
-               fp16 hi/lo solves, FP64 residual of the folded operator, one correction
-               (two for the low Poisson modes)
+               fp16 hi/lo solves, FP64 residual of the folded operator, one correction;
+               the Poisson modes with xlambda < 64 by the unfolded FP64 passes
   mixed_fold_q the same with the residual's products formed exactly from int8 digits on the
                int8 matrix cores (scheme fold16q), summed to FP64 accuracy
 The reported `value` is the fastest path whose result is within 1e-10 (relative,
@@ -258,9 +259,9 @@ def main():
     alams = [workloads.alambda_of(cfg, f) for f in range(F)]
     bounds = workloads.BOUNDS[:F]
     rhs = [torch.from_numpy(x).cuda() for x in fields]
-    # path -> (tier, bar): the FP64 tier is held to the FP64 bar; the folded FP64 solve and the
-    # refined solves to the metric's 1e-10
-    paths = {"fp64": ("fp64", TOL_FP64), "fp64_fold_d": ("fp64", TOL_FP64), "fp64_fold": ("fp64 folded", TOL),
+    # path -> (tier, bar): the FP64 paths are held to the FP64 bar, the refined solves to the
+    # metric's 1e-10
+    paths = {"fp64": ("fp64", TOL_FP64), "fp64_fold_d": ("fp64", TOL_FP64), "fp64_fold": ("fp64", TOL_FP64),
              "mixed": ("mixed", TOL), "mixed_f32": ("mixed", TOL), "mixed_fold": ("mixed", TOL),
              "mixed_fold_q": ("mixed", TOL)}
     sol = {p: [torch.zeros_like(r) for r in rhs] for p in paths}
@@ -285,34 +286,40 @@ def main():
     planes = F * (2 * M - 1)                      # Im at k=0 not solved
     nprod = {"f32": 1, "bf16x1": 1, "bf16x3": 3, "bf16x6": 6}
     g = 2.0 * n ** 3 * planes                    # one unfolded product of every plane
-    gf = 2.0 * (he * he + ho * ho) * n * planes  # one folded product of every plane
+
+    # real planes solved by unfolded and by folded passes on each path: the folded paths give the
+    # Poisson modes with xlambda < 64 (kLowXlam of the library) to the unfolded FP64 passes
+    def nlow(f):
+        return sum(1 for k in range(M) if alams[f] + (cfg.alp * k) ** 2 < 64.0) if bounds[f] == "n" else 0
+
+    def planes_of(k0, k1):
+        return sum(1 if k == 0 else 2 for k in range(k0, k1))
+
+    def plane_split(path):
+        unf = fol = 0
+        for f in range(F):
+            if path in ("fp64", "mixed", "mixed_f32") or (path == "fp64_fold_d" and bounds[f] == "n"):
+                unf += planes_of(0, M)
+            else:
+                unf += planes_of(0, nlow(f))
+                fol += planes_of(nlow(f), M)
+        return unf, fol
 
     def work_per_step(kind, path):
-        """Executed operations of one step of a kernel class and the peak they are held to (the
-        extra correction of the low Poisson modes is not counted)."""
+        """Executed operations of one step of a kernel class and the peak they are held to."""
+        unf, fol = plane_split(path)
+        per = 2.0 * (he * he + ho * ho) * n       # one folded product of one real plane
         if kind.startswith("proj_"):
-            return g, "f64"
+            return 2.0 * n ** 3 * unf, "f64"
         if kind.startswith("fold_"):
-            # folded fields: the Dirichlet ones (fp64_fold_d) or all; the fused middle passes skip
-            # the singular Neumann planes (alambda = 0, mode 0), which the separate passes take
-            # one complex plane (two real planes) at a time
-            nd = sum(1 for c in bounds if c == "d")
-            pf = (nd if path == "fp64_fold_d" else F) * (2 * M - 1)
-            nsing = 0 if path == "fp64_fold_d" else sum(1 for c, al in zip(bounds, alams) if c == "n" and al == 0.0)
-            per = 2.0 * (he * he + ho * ho) * n
-            fused = kt_kinds.get(path, set())
-            if kind == "fold_mid":                 # k_projq: passes 2 and 3 in one launch
-                return 2.0 * per * (pf - nsing), "f64"
-            if kind in ("fold_p2p_div", "fold_p2p") and "fold_mid" in fused:
-                return per * 2 * nsing, "f64"
-            return per * pf, "f64"
+            return (2.0 if kind == "fold_mid" else 1.0) * per * fol, "f64"   # k_projq: passes 2 and 3
         if path in ("mixed_fold", "mixed_fold_q"):
             if kind == "lp_p2p":                   # k_lpq: 4 passes x 3 fp16 products, two solves
-                return 2 * 4 * 3 * gf, "f16"
+                return 2 * 4 * 3 * per * fol, "f16"
             if kind == "resid":
                 if path == "mixed_fold":           # two folded FP64 products
-                    return 2.0 * gf, "f64"
-                return 2 * 18 * gf, "i8"           # two products x 18 int8 digit products
+                    return 2.0 * per * fol, "f64"
+                return 2 * 18 * per * fol, "i8"    # two products x 18 int8 digit products
             return 0.0, None                       # fold / unfold: bandwidth-bound
         if path not in schemes:
             return 0.0, None                       # gauge, boundary: not matrix-core work
@@ -324,7 +331,6 @@ def main():
         return 0.0, None
 
     res = {}
-    kt_kinds = {}                                  # the kernel classes each path launched
     # --path X times only X (a profiled run then holds only X's kernels)
     for path in (paths if args.path == "auto" else (args.path,)):
         for _ in range(args.warmup):
@@ -347,7 +353,6 @@ def main():
             dt_ev = (time.perf_counter() - t0) / args.steps
         if args.events_only:
             dt = dt_ev
-        kt_kinds[path] = {k for k in kt.ms if kt.count[k]}
         classes = {}
         for k in kt.ms:
             if not kt.count[k]:
@@ -391,8 +396,6 @@ def main():
             "mixed_fold", "mixed_fold_q") else "fold" if k.startswith("fold_") else k
         if k == "fold_mid":
             key = "fold_mid"                       # k_projq, its own kernel
-        elif key == "fold" and k in ("fold_p2p_div", "fold_p2p") and "fold_mid" in ks:
-            key = "fold_singular"                  # the singular planes' separate passes
         groups.setdefault(key, []).append(k)
     kernel_names = {"proj": "k_proj (FP64 MFMA projection pass)",
                     "fold": "k_projf (FP64 MFMA projection pass, even/odd folded basis)",

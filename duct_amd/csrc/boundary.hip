@@ -155,83 +155,6 @@ hipError_t launch_gauge(const GaugeArgs& a, const GaugeList& list, hipStream_t s
     return hipGetLastError();
 }
 
-// Gauge on the folded, undivided pass-2 output (quarters (r, c), element
-// (l, k) of a quarter at k NH + l, divided here by lam_r(l) + lam_c(k) - xlambda).  U(a,a) of the unfolded solution is the sum over the
-// quarters of s_r s_c (P_r G_rc P_c^T)(a', a'), a' the folded index of a and
-// s_o = -1 when a lies in the mirrored half; only the ee quarter holds the
-// null mode, so its (io, io) entry is set as in the unfolded gauge.
-template <int LI>   // LI = NH / 64
-__global__ __launch_bounds__(kAuxThreads) void k_gauge_fold(GaugeFoldArgs a, GaugeList list)
-{
-    constexpr int KCH = GaugeShape<LI>::KCH;
-    constexpr int NH = 64 * LI;
-    const int plane = list.plane[blockIdx.x];
-    const int n = a.g.n;
-    const int hh[2] = {(n + 1) / 2, n / 2};
-    double* Gp = a.G + (size_t)plane * 4 * NH * NH;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int fm = plane >> 1;
-    const double xlam = xlambda_of(a.g.alambda[fm / a.g.nmodes], a.g.alp, a.g.mode0 + fm % a.g.nmodes);
-    double part = 0.0;
-    for (int q = 0; q < 4; ++q) {
-        const int r = q >> 1, c = q & 1;
-        const double* G = Gp + (size_t)q * NH * NH;
-        const double* pr = a.prow + r * NH;
-        const double* pc = a.prow + c * NH;
-        const double* lr = a.lam + r * NH;
-        const double* lc = a.lam + c * NH;
-        double pl[LI], ll[LI];
-#pragma unroll
-        for (int h = 0; h < LI; ++h) {
-            pl[h] = pr[lane + 64 * h];
-            ll[h] = lr[lane + 64 * h];
-        }
-        for (int k0 = wave; k0 < hh[c]; k0 += kAuxWaves * KCH) {
-            double gv[KCH][LI], pk[KCH], lk[KCH];
-#pragma unroll
-            for (int t = 0; t < KCH; ++t) {
-                const int k = k0 + kAuxWaves * t, kc = k < hh[c] ? k : 0;
-                pk[t] = pc[kc];
-                lk[t] = lc[kc];
-#pragma unroll
-                for (int h = 0; h < LI; ++h) gv[t][h] = G[(size_t)kc * NH + lane + 64 * h];
-            }
-#pragma unroll
-            for (int t = 0; t < KCH; ++t) {
-                const int k = k0 + kAuxWaves * t;
-#pragma unroll
-                for (int h = 0; h < LI; ++h) {
-                    const int l = lane + 64 * h;
-                    if (k < hh[c] && l < hh[r] && !(q == 0 && l == a.io && k == a.io))
-                        part += pl[h] * (gv[t][h] / (ll[h] + lk[t] - xlam)) * pk[t];
-                }
-            }
-        }
-    }
-    __shared__ double red[kAuxWaves];
-    part = wave_sum(part);
-    if (lane == 0) red[wave] = part;
-    __syncthreads();
-    if (tid == 0) {
-        double valore = 0.0;
-        for (int w = 0; w < kAuxWaves; ++w) valore += red[w];
-        Gp[(size_t)a.io * NH + a.io] = (a.wanted - valore) / (a.pdiag * a.pdiag);
-    }
-}
-
-hipError_t launch_gauge_fold(const GaugeFoldArgs& a, const GaugeList& list, hipStream_t st)
-{
-    if (list.count <= 0) return hipSuccess;
-    const dim3 grid(list.count), block(kAuxThreads);
-    switch (a.NH) {
-        case 64: hipLaunchKernelGGL(k_gauge_fold<1>, grid, block, 0, st, a, list); break;
-        case 128: hipLaunchKernelGGL(k_gauge_fold<2>, grid, block, 0, st, a, list); break;
-        case 256: hipLaunchKernelGGL(k_gauge_fold<4>, grid, block, 0, st, a, list); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 // r -= s p p^T, s = q^T r q, on one part (re or im) of each listed plane.
 __global__ __launch_bounds__(kAuxThreads) void k_nullproj(NullProjArgs a, GaugeList list)
 {

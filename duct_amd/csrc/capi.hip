@@ -66,9 +66,6 @@ struct DevBasis {
     double* ffwd = nullptr;  // folded P^-1, (2 NH) x NH fragment layout
     double* fbwd = nullptr;  // folded P
     double* flam = nullptr;  // eigenvalues per parity, 2 NH
-    double* fprow = nullptr; // Neumann: signed folded rows of P at the gauge node, 2 NH
-    double fpdiag = 0.0;     // Neumann: P_e(a', io_e)
-    int fio = -1;            // Neumann: index of the null mode among the even eigenvectors
     // folded mixed solve (NH = 64 or 128): fp16 hi/lo bases, float eigenvalues, folded S blocks
     bool lq_ready = false;
     void* lq_fwd[2] = {nullptr, nullptr};
@@ -78,8 +75,6 @@ struct DevBasis {
     double* sfold[2] = {nullptr, nullptr};   // centrosymmetric part of D2B, folded e / o, FP64 fragments
     i4v* sdig[2] = {nullptr, nullptr};       // the same blocks as int8 digits (build_sdig)
     int* sexp[2] = {nullptr, nullptr};       // and their row exponents
-    double* nqf = nullptr;                   // Neumann: folded P^-1 row of the null mode
-    double* npf = nullptr;                   // Neumann: folded P column of the null mode
     double pdiag = 0.0, wanted = 0.0;
     int gauge = -1;          // 0-based eigen-index of the null mode (Neumann)
 };
@@ -210,12 +205,11 @@ void free_basis(DevBasis& B)
     free_ptr(B.D2b); free_ptr(B.prow); free_ptr(B.lpf_fwd); free_ptr(B.lpf_bwd); free_ptr(B.lph_fwd);
     free_ptr(B.lph_bwd); free_ptr(B.opfrag); free_ptr(B.qrow); free_ptr(B.pcol);
     free_ptr(B.AfwdT); free_ptr(B.AbwdT); free_ptr(B.ffwd); free_ptr(B.fbwd); free_ptr(B.flam);
-    free_ptr(B.fprow);
     for (int p = 0; p < 2; ++p) {
         free_ptr(B.lq_fwd[p]); free_ptr(B.lq_bwd[p]); free_ptr(B.sfold[p]);
         free_ptr(B.sdig[p]); free_ptr(B.sexp[p]);
     }
-    free_ptr(B.lq_lam); free_ptr(B.nqf); free_ptr(B.npf);
+    free_ptr(B.lq_lam); 
     B = DevBasis{};
 }
 
@@ -361,44 +355,54 @@ int bound_index(char c)
     return -1;
 }
 
-// Folded passes 2 and 3 as two planar launches, with the Neumann gauge between them on the
-// singular planes: X (pass-1 output) -> G (undivided) in W -> X (pass-3 output).
+// The folded solves use the even/odd eigenbases of the centrosymmetric part S = (D2B + J D2B J)/2
+// of the interior operator.  D2B is centrosymmetric only up to rounding (~2e-15 of its norm), and
+// on the ill-conditioned low Poisson modes (xlambda < kLowXlam) S^-1 f and D2B^-1 f differ by up
+// to 4e-11 of the field at m = 257 (1e-12 from xlambda ~ 25 up, 2e-13 from 64 up).  Those
+// leading modes of a Neumann group, the singular k=0 plane among them, are therefore solved
+// with the unfolded FP64 passes, solvediag's own arithmetic; the folded solves take the rest.
+constexpr double kLowXlam = 64.0;
+
+// Number of leading modes of g with xlambda < kLowXlam in some field (xlambda grows with the
+// mode index, so these modes are a prefix).
+int low_mode_prefix(const FieldGroup& g)
+{
+    int k = 0;
+    for (; k < g.nmodes; ++k) {
+        bool low = false;
+        for (int f = 0; f < g.nf; ++f) low = low || xlambda_of(g.alambda[f], g.alp, g.mode0 + k) < kLowXlam;
+        if (!low) break;
+    }
+    return k;
+}
+
+// Modes [k0, k0 + nk) of g as a group of its own.
+FieldGroup mode_range(const FieldGroup& g, int k0, int nk)
+{
+    FieldGroup s = g;
+    s.nmodes = nk;
+    s.mode0 = g.mode0 + k0;
+    const size_t off = (size_t)k0 * g.m * g.m;
+    for (int f = 0; f < g.nf; ++f) {
+        s.in[f] = g.in[f] + off;
+        s.out[f] = g.out[f] + off;
+    }
+    return s;
+}
+
+// Folded passes 2 and 3 as two planar launches: X (pass-1 output) -> G (undivided) in W -> X
+// (pass-3 output, divided on load).
 int run_fold_middle(const FieldGroup& g, const DevBasis& B, double* X, double* W)
 {
     const int NH = B.NH;
-    const bool neu = &B == &L.basis[1];
     ProjFoldArgs a{};
     a.g = g;
     a.lam = B.flam;
     a.ncols = (long)g.nf * g.nmodes * NH;
-    a.gauge = neu ? B.fio : -1;
     {   // pass 2 undivided: the divide is applied by pass 3 as it loads G
         Timed t(PK_F_P2P_DIV);
         a.Afrag = B.ffwd; a.X = X; a.Y = W;
         HIPCHK(launch_proj_fold(NH, PROJ_P2P, a, L.stream));
-    }
-    if (neu) {  // singular planes: |xlambda| <= 1e-10 (solvediag.f90:151)
-        GaugeFoldArgs ga{};
-        ga.G = W; ga.prow = B.fprow; ga.pdiag = B.fpdiag; ga.wanted = B.wanted; ga.lam = B.flam;
-        ga.io = B.fio; ga.NH = NH; ga.g = g;
-        GaugeList gl{};
-        for (int f = 0; f < g.nf; ++f)
-            for (int k = 0; k < g.nmodes; ++k) {
-                if (!(std::fabs(xlambda_of(g.alambda[f], g.alp, g.mode0 + k)) <= 1.0e-10)) continue;
-                for (int part = 0; part < 2; ++part) {
-                    if (part == 1 && g.mode0 + k == 0) continue;
-                    gl.plane[gl.count++] = 2 * (f * g.nmodes + k) + part;
-                    if (gl.count == kMaxGaugePlanes) {
-                        Timed t(PK_GAUGE);
-                        HIPCHK(launch_gauge_fold(ga, gl, L.stream));
-                        gl.count = 0;
-                    }
-                }
-            }
-        if (gl.count) {
-            Timed t(PK_GAUGE);
-            HIPCHK(launch_gauge_fold(ga, gl, L.stream));
-        }
     }
     {
         Timed t(PK_F_P2P);
@@ -408,7 +412,8 @@ int run_fold_middle(const FieldGroup& g, const DevBasis& B, double* X, double* W
     return OK;
 }
 
-// The four passes in the even/odd folded basis, then the boundary.  bmode is a BoundaryMode.
+// The four passes in the even/odd folded basis, then the boundary.  bmode is a BoundaryMode.  g holds
+// no singular plane (run_group gives the low Poisson modes to the unfolded passes).
 int run_group_fold(const FieldGroup& g, int bidx, int bmode)
 {
     DevBasis& B = L.basis[bidx];
@@ -421,7 +426,6 @@ int run_group_fold(const FieldGroup& g, int bidx, int bmode)
     a.g = g;
     a.lam = B.flam;
     a.ncols = cols;
-    a.gauge = bidx == 1 ? B.fio : -1;
     {
         Timed t(PK_F_C2P);
         a.Afrag = B.ffwd; a.X = nullptr; a.Y = L.plan[0];
@@ -432,24 +436,8 @@ int run_group_fold(const FieldGroup& g, int bidx, int bmode)
             Timed t(PK_F_MID);
             ProjQuarterArgs q{};
             q.fwd = B.ffwd; q.bwd = B.fbwd; q.lam = B.flam; q.X = L.plan[0];
-            q.nplanes = planes; q.gauge = a.gauge; q.g = g;
+            q.nplanes = planes; q.g = g;
             HIPCHK(launch_proj_quarter(NH, q, L.stream));
-        }
-        if (a.gauge >= 0) {   // the singular planes it skips: the separate passes and the gauge, per plane
-            for (int f = 0; f < g.nf; ++f)
-                for (int k = 0; k < g.nmodes; ++k) {
-                    if (!(std::fabs(xlambda_of(g.alambda[f], g.alp, g.mode0 + k)) <= 1.0e-10)) continue;
-                    FieldGroup g1 = g;
-                    g1.nf = 1;
-                    g1.nmodes = 1;
-                    g1.mode0 = g.mode0 + k;
-                    g1.alambda[0] = g.alambda[f];
-                    g1.in[0] = g.in[f] + (size_t)k * g.m * g.m;
-                    g1.out[0] = g.out[f] + (size_t)k * g.m * g.m;
-                    const size_t off = (size_t)(f * g.nmodes + k) * 2 * 4 * NH * NH;
-                    int st1 = run_fold_middle(g1, B, L.plan[0] + off, L.plan[1] + off);
-                    if (st1) return st1;
-                }
         }
     } else {
         int st1 = run_fold_middle(g, B, L.plan[0], L.plan[1]);
@@ -470,12 +458,26 @@ int run_group_fold(const FieldGroup& g, int bidx, int bmode)
     return OK;
 }
 
+int run_group_unfolded(const FieldGroup& g, int bidx, int bmode);
+
 int run_group(const FieldGroup& g, int bidx, int bmode)
 {
     DevBasis& B = L.basis[bidx];
     if (!B.ready) return fail(ERR_NOT_SETUP, "duct_setup_ not called for bound '%c'", bidx ? 'n' : 'd');
     if (g.m != B.m) return fail(ERR_ARG, "m=%d but the '%c' basis was set up for m=%d", g.m, bidx ? 'n' : 'd', B.m);
-    if (B.NH > 0 && L.fold > bidx) return run_group_fold(g, bidx, bmode);
+    if (!(B.NH > 0 && L.fold > bidx)) return run_group_unfolded(g, bidx, bmode);
+    // folded, except the low Poisson modes of a Neumann group (kLowXlam)
+    const int klow = bidx == 1 ? low_mode_prefix(g) : 0;
+    int st;
+    if (klow > 0 && (st = run_group_unfolded(mode_range(g, 0, klow), bidx, bmode))) return st;
+    if (klow < g.nmodes) return run_group_fold(mode_range(g, klow, g.nmodes - klow), bidx, bmode);
+    return OK;
+}
+
+// The four FP64 passes of solvediag in the unfolded eigenbasis, the gauge, the boundary.
+int run_group_unfolded(const FieldGroup& g, int bidx, int bmode)
+{
+    DevBasis& B = L.basis[bidx];
     const int NP = B.NP;
     const long cplx_cols = (long)g.nf * g.nmodes * NP;
     const long planes = (long)g.nf * g.nmodes * 2;
@@ -702,26 +704,24 @@ int run_group_lp(const FieldGroup& g, int bidx, int scheme, int accumulate, doub
 // stats (host, may be null): [(niter+1) x nf x 2] = max|r_it|, max|f| per field,
 // including the residual after the last correction.
 // Folded mixed solve of one boundary group (mixed_fold.hip): fold, fused low-precision
-// solve, niter x (folded FP64 residual, null projection, fused solve), one more
-// correction of the low Poisson modes (|xlambda| < kLowModes), unfold-and-sum,
-// FP64 gauge, boundary.  stats as in solve_fields_mixed (the last entry is the
+// solve, niter x (folded FP64 residual, fused solve), unfold-and-sum, boundary.  The group
+// holds no low Poisson mode (run_mixed_fold_split gives those to the unfolded FP64 passes),
+// so no plane of it is singular.  stats as in solve_fields_mixed (the last entry is the
 // residual after the final correction, computed only when asked for).
-constexpr double kLowModes = 64.0;
-
 int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long long* stats, const int* fidx,
                          int nf_all, bool i8)
 {
     DevBasis& B = L.basis[bidx];
     const int NH = B.NH;
     const size_t arr = (size_t)g.nf * g.nmodes * 2 * 4 * NH * NH;   // floats per folded array
-    // F, R, U0 (FP64-residual form) or Flo (int8 form: U0 is in digits), D1..Dniter, Dlow
-    const int narr = 2 + 1 + niter + 1;
+    // F, R, U0 (FP64-residual form) or Flo (int8 form: U0 is in digits), D1..Dniter
+    const int narr = 2 + 1 + niter;
     int st = ensure_buf(L.mfold, L.mfold_bytes, arr * narr * sizeof(float));
     if (st) return st;
     float* F = L.mfold;
     float* R = L.mfold + arr;
-    float* D[kMaxFoldSources + 1];
-    for (int t = 0; t < niter + 2; ++t) D[t] = L.mfold + (size_t)(2 + t) * arr;   // D[0] = U0, D[niter+1] = low
+    float* D[kMaxFoldSources];
+    for (int t = 0; t < niter + 1; ++t) D[t] = L.mfold + (size_t)(2 + t) * arr;   // D[0] = U0
     float* Flo = i8 ? D[0] : nullptr;
     unsigned char* Ucol = nullptr;
     unsigned char* Urow = nullptr;
@@ -743,55 +743,11 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
         la.sfwd[p] = B.lq_sfwd[p]; la.sbwd[p] = B.lq_sbwd[p];
     }
     la.lam = B.lq_lam;
-    la.gauge = bidx == 1 ? B.fio : -1;
     auto lpq = [&](const FieldGroup& gg, const float* in, float* out) -> int {
         Timed t(PK_LP_P2P);
         la.g = gg; la.X = in; la.Y = out;
         la.Ucol = nullptr; la.Urow = nullptr; la.gexp = nullptr;
         HIPCHK(launch_lpq(NH, la, L.stream));
-        return OK;
-    };
-    auto nullproj = [&](const FieldGroup& gg, unsigned long long* sp, size_t off) -> int {
-        NullFoldArgs np{};
-        np.R = R + off; np.qf = B.nqf; np.pf = B.npf; np.stats = sp; np.NH = NH; np.nf = gg.nf;
-        for (int i = 0; i < gg.nf; ++i) np.slot[i] = fidx[i];
-        GaugeList gl{};
-        for (int k = 0; k < gg.nmodes; ++k)
-            for (int f = 0; f < gg.nf; ++f) {
-                if (!(std::fabs(xlambda_of(gg.alambda[f], gg.alp, gg.mode0 + k)) <= 1.0e-10)) continue;
-                for (int part = 0; part < 2; ++part) {
-                    if (part == 1 && gg.mode0 + k == 0) continue;
-                    gl.plane[gl.count++] = 2 * (k * gg.nf + f) + part;
-                    if (gl.count == kMaxGaugePlanes) {
-                        Timed t(PK_GAUGE);
-                        HIPCHK(launch_nullproj_fold(np, gl, L.stream));
-                        gl.count = 0;
-                    }
-                }
-            }
-        if (gl.count) {
-            Timed t(PK_GAUGE);
-            HIPCHK(launch_nullproj_fold(np, gl, L.stream));
-        }
-        return OK;
-    };
-    // residual of the modes of gg (a prefix, or a suffix starting at plane offset `off`) with u = sum of D[0..nsrc)
-    auto resid = [&](const FieldGroup& gg, int nsrc, unsigned long long* sp, size_t off) -> int {
-        ResidFoldArgs ra{};
-        ra.g = gg;
-        for (int t = 0; t < nsrc; ++t) ra.src[t] = D[t] + off;
-        ra.nsrc = nsrc;
-        ra.R = R + off;
-        ra.Sfrag[0] = B.sfold[0]; ra.Sfrag[1] = B.sfold[1];
-        ra.stats = sp;
-        for (int i = 0; i < gg.nf; ++i) ra.slot[i] = fidx[i];
-        ra.neumann = bidx == 1;
-        ra.NH = NH;
-        {
-            Timed t(PK_RESID);
-            HIPCHK(launch_resid_fold(ra, L.stream));
-        }
-        if (bidx == 1 && (st = nullproj(gg, sp, off))) return st;
         return OK;
     };
     if (i8) {
@@ -809,16 +765,13 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
         ra.R = R;
         ra.Sdig[0] = B.sdig[0]; ra.Sdig[1] = B.sdig[1];
         ra.Sexp[0] = B.sexp[0]; ra.Sexp[1] = B.sexp[1];
-        unsigned long long* sp = stats ? stats : nullptr;
-        ra.stats = sp;
+        ra.stats = stats;
         for (int i = 0; i < g.nf; ++i) ra.slot[i] = fidx[i];
-        ra.neumann = bidx == 1;
         ra.NH = NH;
         {
             Timed t(PK_RESID);
             HIPCHK(launch_resid_i8(ra, L.stream));
         }
-        if (bidx == 1 && (st = nullproj(g, sp, 0))) return st;
         if ((st = lpq(g, R, D[1]))) return st;
         Timed t(PK_LP_P2C);
         const float* src[kMaxFoldSources] = {D[1], nullptr, nullptr, nullptr};
@@ -826,106 +779,38 @@ int run_group_mixed_fold(const FieldGroup& g, int bidx, int niter, unsigned long
     } else {
         if ((st = lpq(g, F, D[0]))) return st;
         for (int it = 0; it < niter; ++it) {
-            if ((st = resid(g, it + 1, stats ? stats + (size_t)2 * nf_all * it : nullptr, 0))) return st;
+            ResidFoldArgs ra{};
+            ra.g = g;
+            for (int t = 0; t <= it; ++t) ra.src[t] = D[t];
+            ra.nsrc = it + 1;
+            ra.R = R;
+            ra.Sfrag[0] = B.sfold[0]; ra.Sfrag[1] = B.sfold[1];
+            ra.stats = stats ? stats + (size_t)2 * nf_all * it : nullptr;
+            for (int i = 0; i < g.nf; ++i) ra.slot[i] = fidx[i];
+            ra.NH = NH;
+            {
+                Timed t(PK_RESID);
+                HIPCHK(launch_resid_fold(ra, L.stream));
+            }
             if ((st = lpq(g, R, D[it + 1]))) return st;
         }
         Timed t(PK_LP_P2C);
         const float* src[kMaxFoldSources] = {D[0], niter >= 1 ? D[1] : nullptr, niter >= 2 ? D[2] : nullptr, nullptr};
         HIPCHK(launch_unfold_sum(g, NH, src, niter + 1, 0, L.stream, nullptr, nullptr, bidx == 0));
     }
-    // The refinement above converges to S^-1 f, S the centrosymmetric part of D2B.  On the
-    // ill-conditioned low Poisson modes (xlambda < kLowModes) that differs from D2B^-1 f by up
-    // to ~1e-10, so those modes get one more correction from the residual of D2B itself
-    // (k_colloc on the unfolded FP64 solution, solvediag's own operator).
-    int klow = 0;
-    if (bidx == 1 && niter >= 1) {
-        while (klow < g.nmodes) {
-            bool low = false;
-            for (int f = 0; f < g.nf; ++f)
-                low = low || xlambda_of(g.alambda[f], g.alp, g.mode0 + klow) < kLowModes;
-            if (!low) break;
-            ++klow;
-        }
-    }
-    auto resid_full = [&](const FieldGroup& gg, unsigned long long* sp) -> int {
-        const size_t fb = (size_t)gg.nmodes * gg.m * gg.m;
-        if ((st = ensure_buf(L.rfield, L.rfield_bytes, fb * gg.nf * sizeof(d2v)))) return st;
+    if (stats) {   // the residual of the returned solution, by the unfolded FP64 residual kernel
+        const size_t fb = (size_t)g.nmodes * g.m * g.m;
+        if ((st = ensure_buf(L.rfield, L.rfield_bytes, fb * g.nf * sizeof(d2v)))) return st;
         ResidArgs ra{};
-        ra.g = gg;
+        ra.g = g;
         ra.Dfrag = B.opfrag;
         ra.NP = B.NP;
-        ra.stats = sp;
+        ra.stats = stats + (size_t)2 * nf_all * niter;
         ra.neumann = bidx == 1;
-        for (int i = 0; i < gg.nf; ++i) { ra.r[i] = L.rfield + (size_t)i * fb; ra.slot[i] = fidx[i]; }
-        {
-            Timed t(PK_RESID);
-            HIPCHK(launch_resid(ra, L.stream));
-        }
-        if (bidx == 1) {
-            NullProjArgs np{};
-            np.q = B.qrow; np.p = B.pcol; np.stats = sp;
-            np.nmodes = gg.nmodes; np.m = gg.m; np.n = gg.n;
-            for (int i = 0; i < gg.nf; ++i) { np.r[i] = ra.r[i]; np.slot[i] = fidx[i]; }
-            GaugeList gl{};
-            for (int f = 0; f < gg.nf; ++f)
-                for (int k = 0; k < gg.nmodes; ++k) {
-                    if (!(std::fabs(xlambda_of(gg.alambda[f], gg.alp, gg.mode0 + k)) <= 1.0e-10)) continue;
-                    for (int part = 0; part < 2; ++part) {
-                        if (part == 1 && gg.mode0 + k == 0) continue;
-                        gl.plane[gl.count++] = 2 * (f * gg.nmodes + k) + part;
-                        if (gl.count == kMaxGaugePlanes) {
-                            Timed t(PK_GAUGE);
-                            HIPCHK(launch_nullproj(np, gl, L.stream));
-                            gl.count = 0;
-                        }
-                    }
-                }
-            if (gl.count) {
-                Timed t(PK_GAUGE);
-                HIPCHK(launch_nullproj(np, gl, L.stream));
-            }
-        }
-        return OK;
-    };
-    if (klow > 0) {
-        FieldGroup glow = g;          // g.in = f, g.out = the current solution
-        glow.nmodes = klow;
-        if ((st = resid_full(glow, nullptr))) return st;
-        FieldGroup gr = glow;         // the residual fields as the right-hand side
-        const size_t fb = (size_t)klow * g.m * g.m;
-        for (int i = 0; i < g.nf; ++i) gr.in[i] = L.rfield + (size_t)i * fb;
-        {
-            Timed t(PK_LP_C2P);
-            HIPCHK(launch_fold32(gr, NH, F, L.stream));
-        }
-        if ((st = lpq(gr, F, D[niter + 1]))) return st;
-        Timed t(PK_LP_P2C);
-        const float* s2[1] = {D[niter + 1]};
-        HIPCHK(launch_unfold_sum(glow, NH, s2, 1, 1, L.stream));
+        for (int i = 0; i < g.nf; ++i) { ra.r[i] = L.rfield + (size_t)i * fb; ra.slot[i] = fidx[i]; }
+        Timed t(PK_RESID);
+        HIPCHK(launch_resid(ra, L.stream));
     }
-    if (bidx == 1) {   // gauge node U(iofzero+1, iofzero+1) = wanted on the singular planes
-        GaugeFixArgs gf{};
-        gf.g = g; gf.p = B.pcol; gf.wanted = B.wanted; gf.a = B.gauge;
-        GaugeList gl{};
-        for (int k = 0; k < g.nmodes; ++k)
-            for (int f = 0; f < g.nf; ++f) {
-                if (!(std::fabs(xlambda_of(g.alambda[f], g.alp, g.mode0 + k)) <= 1.0e-10)) continue;
-                for (int part = 0; part < 2; ++part) {
-                    if (part == 1 && g.mode0 + k == 0) continue;
-                    gl.plane[gl.count++] = 2 * (k * g.nf + f) + part;
-                    if (gl.count == kMaxGaugePlanes) {
-                        Timed t(PK_GAUGE);
-                        HIPCHK(launch_gauge_fix(gf, gl, L.stream));
-                        gl.count = 0;
-                    }
-                }
-            }
-        if (gl.count) {
-            Timed t(PK_GAUGE);
-            HIPCHK(launch_gauge_fix(gf, gl, L.stream));
-        }
-    }
-    if (stats && (st = resid_full(g, stats + (size_t)2 * nf_all * niter))) return st;
     if (bidx == 1) {   // (the Dirichlet boundary was zeroed by the unfold)
         Timed t(PK_BOUNDARY);
         BoundaryArgs ba{};
@@ -959,6 +844,19 @@ struct SideLane {
 
 int solve_group_mixed(const FieldGroup& g, int bidx, bool folded, bool i8, int scheme0, int scheme1, int niter,
                       int nstat, int nf, const int* fidx);
+
+// The folded refined solve of one boundary group; the low Poisson modes of a Neumann group
+// (kLowXlam), the singular k=0 plane among them, take the unfolded FP64 passes instead.
+int run_mixed_fold_split(const FieldGroup& g, int bidx, int niter, unsigned long long* stats, const int* fidx,
+                         int nf_all, bool i8)
+{
+    const int klow = bidx == 1 ? low_mode_prefix(g) : 0;
+    int st;
+    if (klow > 0 && (st = run_group_unfolded(mode_range(g, 0, klow), bidx, BND_NEUMANN))) return st;
+    if (klow < g.nmodes)
+        return run_group_mixed_fold(mode_range(g, klow, g.nmodes - klow), bidx, niter, stats, fidx, nf_all, i8);
+    return OK;
+}
 
 int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* sol, const char* bounds,
                        const double* alambda, int m, int nmodes, int ib, double alp, int scheme0, int scheme1,
@@ -1023,7 +921,7 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
         int st_side;
         {
             SideLane lane;
-            st_side = run_group_mixed_fold(gs[1], 1, niter, nstat ? L.stats : nullptr, fidxs[1], nf, i8);
+            st_side = run_mixed_fold_split(gs[1], 1, niter, nstat ? L.stats : nullptr, fidxs[1], nf, i8);
         }
         HIPCHK(hipEventRecord(L.ev_join, L.side));
         HIPCHK(hipStreamWaitEvent(L.stream, L.ev_join, 0));
@@ -1054,7 +952,7 @@ int solve_group_mixed(const FieldGroup& g, int bidx, bool folded, bool i8, int s
     const int m = g.m, nmodes = g.nmodes;
     int st;
     {
-        if (folded) return run_group_mixed_fold(g, bidx, niter, nstat ? L.stats : nullptr, fidx, nf, i8);
+        if (folded) return run_mixed_fold_split(g, bidx, niter, nstat ? L.stats : nullptr, fidx, nf, i8);
         if ((st = run_group_lp(g, bidx, scheme0, 0, B.wanted))) return st;
         ResidArgs ra{};
         ra.g = g;
@@ -1626,29 +1524,12 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
             std::vector<int> pos(n);
             bool ok = build_fold_bases(P, Pinv, lam, n, NH, ff.data(), fb.data(), fl.data(), pos.data(),
                                        op.data()) == 0;
-            std::vector<double> fprow((size_t)2 * NH, 0.0);
-            if (ok && bidx == 1) {   // the null mode must be even; the gauge node a = iofzero-1
-                const int io = *iofzero - 1, he = (n + 1) / 2;
-                ok = pos[io] >= 0;
-                if (ok) {
-                    const int a0 = io < he ? io : n - 1 - io, a1 = n - 1 - a0;
-                    const double so = io < he ? 1.0 : -1.0;
-                    for (int k = 0; k < n; ++k) {
-                        const double pa = P[a0 + (size_t)k * n], pb = P[a1 + (size_t)k * n];
-                        if (pos[k] >= 0) fprow[pos[k]] = 0.5 * (pa + pb);
-                        else fprow[NH + (-1 - pos[k])] = so * 0.5 * (pa - pb);
-                    }
-                    B.fio = pos[io];
-                    B.fpdiag = fprow[pos[io]];
-                }
-            }
             if (ok) {
                 ff.resize(ff.size() + kFoldFragSlack, 0.0);
                 fb.resize(fb.size() + kFoldFragSlack, 0.0);
                 if ((st = up(B.ffwd, ff.data(), ff.size()))) return st;
                 if ((st = up(B.fbwd, fb.data(), fb.size()))) return st;
                 if ((st = up(B.flam, fl.data(), fl.size()))) return st;
-                if (bidx == 1 && (st = up(B.fprow, fprow.data(), fprow.size()))) return st;
                 B.NH = NH;
             }
         }
@@ -1711,22 +1592,7 @@ int duct_setup_(const char* bound, const int* m_, const double* P, const double*
                     if ((st = upraw2(dv, se.data(), se.size() * sizeof(int)))) return st;
                     B.sexp[par] = (int*)dv;
                 }
-                bool ok = true;
-                if (bidx == 1) {
-                    const int io = *iofzero - 1;
-                    ok = pos[io] >= 0;
-                    if (ok) {
-                        const int fio = pos[io];
-                        std::vector<double> qf(NH), pf(NH);
-                        for (int j = 0; j < NH; ++j) {
-                            qf[j] = Af[(size_t)fio + (size_t)j * 2 * NH];
-                            pf[j] = Ab[(size_t)j + (size_t)fio * 2 * NH];
-                        }
-                        if ((st = up(B.nqf, qf.data(), NH))) return st;
-                        if ((st = up(B.npf, pf.data(), NH))) return st;
-                    }
-                }
-                B.lq_ready = ok;
+                B.lq_ready = true;
             }
         }
         if (bidx == 1) {

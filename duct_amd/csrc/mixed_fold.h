@@ -36,17 +36,21 @@ struct LpqArgs {
     const float* X;         // folded FP32 input
     float* Y;               // folded FP32 output
     // Digit output (Ucol non-null, Y unused): the solution U of each quarter is rounded to the
-    // grid 2^(g-24) Z with 2^g > max|U| / 0.49 (g per quarter, in gexp[quarter]); Ucol / Urow
-    // receive the three signed base-256 digits of Z = c1 2^16 + c2 2^8 + c3, |c| <= 127, as
-    // bytes: Ucol[quarter][d][z'][y'] and the transposed Urow[quarter][d][y'][z'] (d = 0 for c1).
+    // grid 2^(g - 8 kUDig) Z with 2^g > max|U| / 0.49 (g per quarter, in gexp[quarter]); Ucol /
+    // Urow receive the kUDig signed base-256 digits of Z = c1 256^(kUDig-1) + ... + c_kUDig,
+    // |c| <= 127, as bytes: Ucol[quarter][d][z'][y'] and the transposed Urow[quarter][d][y'][z']
+    // (d = 0 for c1).
     unsigned char* Ucol;
     unsigned char* Urow;
     int* gexp;
-    int gauge;              // index of the Neumann null mode among the even eigenvectors, -1 if none
-    FieldGroup g;           // planes, xlambda per plane
+    FieldGroup g;           // planes, xlambda per plane (no singular plane: |den| > 0)
 };
 hipError_t launch_lpq(int NH, const LpqArgs& a, hipStream_t st);
-constexpr int kUDig = 3;    // digits of a quantised quarter solution
+// Digits of the quantised first solution U0 (24 bits of each quarter's maximum).  Its rounding
+// is white noise, which L amplifies into the residual by up to its largest eigenvalue, and the
+// correction solve's error grows with |r| cond(L), not with the correction: with 16 bits (two
+// digits) the refined Helmholtz field at m = 200 ended 1.1e-10 from the oracle.
+constexpr int kUDig = 3;
 // fp16 hi/lo fragments of an NH x NH block B (row i, column k, column-major with leading dim ld);
 // returns the scale s (B * s has max magnitude in [2^11, 2^12); hi = fp16(B s), lo = fp16(B s - hi)).
 float build_lpq_basis(const double* B, int ld, int NH, std::vector<unsigned short>& frag);
@@ -61,7 +65,6 @@ struct ResidFoldArgs {
     const double* Sfrag[2];             // folded S_e, S_o, FP64 fragment layout (build_frag, NP = NH)
     unsigned long long* stats;          // max|r| (bit patterns) at stats[2 slot[f]], max|f| at +1, or null
     int slot[kMaxFields];
-    int neumann;                        // singular planes' max|r| is left to the null projection
     int NH;
 };
 hipError_t launch_resid_fold(const ResidFoldArgs& a, hipStream_t st);
@@ -87,7 +90,6 @@ struct ResidI8Args {
     const int* Sexp[2];                 // their row exponents, NH each
     unsigned long long* stats;          // as ResidFoldArgs
     int slot[kMaxFields];
-    int neumann;
     int NH;
 };
 hipError_t launch_resid_i8(const ResidI8Args& a, hipStream_t st);
@@ -95,28 +97,5 @@ hipError_t launch_resid_i8(const ResidI8Args& a, hipStream_t st);
 // layout of the int8 products: frag[((rt kSDig + t) KC + kc) 64 + lane] holds the 16 bytes
 // d_(t+1)(rt 16 + lane%16, kc 64 + (lane/16) 16 + b), KC = NH/64; exps[row] = e.
 void build_sdig(const double* B, int NH, std::vector<i4v>& frag, std::vector<int>& exps);
-
-// Singular (Neumann, |xlambda| <= 1e-10) planes: r_ee -= (qf^T r_ee qf) pf pf^T, the null-space
-// part of the residual removed in FP64; stats = max|r| over the four quarters afterwards.
-struct NullFoldArgs {
-    float* R;
-    const double* qf;                   // folded row of P^-1 for the null mode, NH
-    const double* pf;                   // folded null column of P, NH
-    unsigned long long* stats;
-    int slot[kMaxFields];
-    int NH, nf;
-};
-hipError_t launch_nullproj_fold(const NullFoldArgs& a, const GaugeList& list, hipStream_t st);
-
-// Gauge on the unfolded FP64 solution of singular planes: u += c p p^T with
-// c = (wanted - u(a,a)) / p(a)^2, p the null column of P (solvediag.f90:161-206 sets
-// the same node to the same value).
-struct GaugeFixArgs {
-    FieldGroup g;                       // g.out = solution fields
-    const double* p;                    // P(:, iofzero), n
-    double wanted;
-    int a;                              // 0-based interior gauge node (iofzero - 1)
-};
-hipError_t launch_gauge_fix(const GaugeFixArgs& a, const GaugeList& list, hipStream_t st);
 
 }  // namespace duct

@@ -4,9 +4,9 @@
 // D2B_z - xlambda) runs in the parity-folded basis of proj_fold.hip:
 //   F  = fold(f)                       FP32 quarters                 k_fold32
 //   U0 = S~(F)                         fused low-precision solve      k_lpq
-//   R  = f - S U0                      FP64 residual, folded          k_resid_fold (+ k_nullproj_fold)
+//   R  = f - S U0                      FP64 residual, folded          k_resid_fold
 //   D1 = S~(R)                         fused low-precision solve      k_lpq
-//   u  = unfold(U0 + D1)               FP64                           k_unfold_sum (+ k_gauge_fix)
+//   u  = unfold(U0 + D1)               FP64                           k_unfold_sum
 // S is the centrosymmetric part (D2B + J D2B J)/2 of the interior operator: its
 // folded blocks are exactly block-diagonal, so the residual costs two n^3/2
 // products per plane.  D2B itself is centrosymmetric up to rounding (~2e-15 of
@@ -140,12 +140,12 @@ __global__ __launch_bounds__(256) void k_unfold_sum(UnfoldArgs a)
             double s = 0.0;
             // every input is read once and the output is not read back here: nontemporal
             // accesses keep them from displacing cached data (~8% of this kernel's time)
-            if (DIG) {   // U = 2^(g-24) (c1 2^16 + c2 2^8 + c3), exact
+            if (DIG) {   // U = 2^(g - 8 kUDig) (c1 256^(kUDig-1) + ... + c_kUDig), exact
                 const unsigned char* ud = a.udig + qi * kUDig * NN + (size_t)zq * NH + yq;
-                const int Z = (int)(signed char)__builtin_nontemporal_load(ud) * 65536 +
-                              (int)(signed char)__builtin_nontemporal_load(ud + NN) * 256 +
-                              (int)(signed char)__builtin_nontemporal_load(ud + 2 * NN);
-                s = ldexp((double)Z, a.gexp[qi] - 24);
+                int Z = 0;
+#pragma unroll
+                for (int d = 0; d < kUDig; ++d) Z = Z * 256 + (int)(signed char)__builtin_nontemporal_load(ud + d * NN);
+                s = ldexp((double)Z, a.gexp[qi] - 8 * kUDig);
             }
             for (int t = 0; t < a.nsrc; ++t) s += (double)__builtin_nontemporal_load(&a.src[t][o]);
             V[k][part] = s;
@@ -283,7 +283,6 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
     }
     const double xl64 = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
     const float xlam = (float)xl64;
-    const bool gauge = a.gauge >= 0 && q == 0 && fabs(xl64) <= 1.0e-10;
 
     auto block_max = [&](float v) -> float {   // v >= 0
         v = wave_max_nonneg(v);
@@ -417,17 +416,6 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
 #pragma unroll
                     for (int e = 0; e < 4; ++e)   // rcp: ~1 ulp, inside the split's 2^-22
                         am[x][y][e] *= __builtin_amdgcn_rcpf(lr[x][e] + lc[y]);
-            if (gauge) {   // the singular plane's null-mode entry (den = 0) is 0
-#pragma unroll
-                for (int x = 0; x < TJ; ++x)
-#pragma unroll
-                    for (int y = 0; y < TI; ++y)
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            const int j = j0 + x * 16 + l4 * 4 + e, i = i0 + y * 16 + l16;
-                            if (j == a.gauge && i == a.gauge) am[x][y][e] = 0.f;
-                        }
-            }
         }
 #pragma unroll
         for (int x = 0; x < TJ; ++x)
@@ -475,11 +463,11 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
                 }
                 return;
             }
-            // digit form: Z = rint(U 2^(24-g)), |Z| < 0.49 2^24, as signed base-256 digits
+            // digit form: Z = rint(U 2^(8 kUDig - g)), |Z| < 0.49 256^kUDig, as signed base-256 digits
             const float gmx = block_max(mx);   // its barrier: every wave is done reading Xs
             int ge = 0;
             if (gmx > 0.f && gmx < 3.0e38f) (void)frexpf(gmx / 0.49f, &ge);
-            const float sc = ldexpf(1.f, 24 - ge);
+            const float sc = ldexpf(1.f, 8 * kUDig - ge);
             // The digit planes go out through LDS (Xs is free now): each lane's words are written
             // to a staging image of the quarter's three planes (rows of NH bytes, 16 bytes of
             // padding against bank conflicts), then stored as whole rows, 16 bytes per lane, so
@@ -507,11 +495,13 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_lpq(LpqArgs a)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         int Z = __float2int_rn(am[x][y][e] * sc);
-                        const int c3 = ((Z + 128) & 255) - 128;
-                        Z = (Z - c3) >> 8;
-                        const int c2 = ((Z + 128) & 255) - 128;
-                        const int c1 = (Z - c2) >> 8;
-                        const int cd[kUDig] = {c1, c2, c3};
+                        int cd[kUDig];   // cd[0] most significant
+#pragma unroll
+                        for (int d = kUDig - 1; d >= 1; --d) {
+                            cd[d] = ((Z + 128) & 255) - 128;
+                            Z = (Z - cd[d]) >> 8;
+                        }
+                        cd[0] = Z;
 #pragma unroll
                         for (int d = 0; d < kUDig; ++d) w[x][y][d] |= (unsigned)(cd[d] & 255) << (8 * e);
                     }
@@ -733,7 +723,6 @@ __global__ __launch_bounds__(256, 2) void k_resid_fold(ResidFoldArgs a, long ncp
     // epilogue: D[i = z][j = y] in lane j + 16 (i%4), element i/4
     const double xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
     const bool zim = (g.mode0 + mode) == 0;
-    const bool singular = a.neumann && fabs(xlam) <= 1.0e-10;
     const d2v* F = g.in[f] + (size_t)mode * m * m;
     const double sr = r ? -1.0 : 1.0, sc = c ? -1.0 : 1.0;
     double rmx = 0.0, fmx = 0.0;
@@ -781,7 +770,7 @@ __global__ __launch_bounds__(256, 2) void k_resid_fold(ResidFoldArgs a, long ncp
                     const d2v lu = {acc[zt][yt][0][e], acc[zt][yt][1][e]};
                     rv = fv - (lu - xlam * u);
                     if (zim) rv[1] = 0.0;
-                    if (!singular) rmx = fmax(rmx, fmax(fabs(rv[0]), fabs(rv[1])));
+                    rmx = fmax(rmx, fmax(fabs(rv[0]), fabs(rv[1])));
                     fmx = fmax(fmx, fmax(fabs(fv[0]), zim ? 0.0 : fabs(fv[1])));
                 }
 #pragma unroll
@@ -816,88 +805,6 @@ hipError_t launch_resid_fold(const ResidFoldArgs& a, hipStream_t st)
         case 4: hipLaunchKernelGGL(k_resid_fold<4>, grid, block, 0, st, a, ncplx); break;
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-constexpr int kAux = 1024;
-
-__device__ __forceinline__ double block_sum(double v, double* red)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    double s = 0.0;
-    for (int w = 0; w < kAux / 64; ++w) s += red[w];
-    return s;
-}
-
-__global__ __launch_bounds__(kAux) void k_nullproj_fold(NullFoldArgs a, GaugeList list)
-{
-    const int plane = list.plane[blockIdx.x];        // 2 fmp + part
-    const int NH = a.NH, tid = threadIdx.x;
-    float* Re = a.R + (size_t)plane * 4 * NH * NH;   // quarter ee of the plane
-    __shared__ double red[kAux / 64];
-    double s = 0.0;
-    for (int e = tid; e < NH * NH; e += kAux) {
-        const int z = e / NH, y = e % NH;
-        s += a.qf[y] * (double)Re[e] * a.qf[z];
-    }
-    s = block_sum(s, red);
-    double mx = 0.0;
-    for (int e = tid; e < NH * NH; e += kAux) {
-        const int z = e / NH, y = e % NH;
-        const float v = (float)((double)Re[e] - s * a.pf[y] * a.pf[z]);
-        Re[e] = v;
-        mx = fmax(mx, fabs((double)v));
-    }
-    for (int e = tid; e < 3 * NH * NH; e += kAux) mx = fmax(mx, fabs((double)Re[(size_t)NH * NH + e]));
-    if (a.stats) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-        if ((tid & 63) == 0) {
-            const int f = (int)((plane >> 1) % a.nf);
-            atomic_max_pos(a.stats + 2 * a.slot[f], mx);
-        }
-    }
-}
-
-hipError_t launch_nullproj_fold(const NullFoldArgs& a, const GaugeList& list, hipStream_t st)
-{
-    if (list.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nullproj_fold, dim3(list.count), dim3(kAux), 0, st, a, list);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(kAux) void k_gauge_fix(GaugeFixArgs a, GaugeList list)
-{
-    const FieldGroup& g = a.g;
-    const int plane = list.plane[blockIdx.x];        // 2 fmp + part
-    const long fmp = plane >> 1;
-    const int part = plane & 1;
-    int f, mode;
-    plane_decode(g, fmp, f, mode);
-    const int m = g.m, n = g.n, io = a.a;
-    double* U = (double*)(g.out[f] + (size_t)mode * m * m) + part;   // stride 2 doubles
-    __shared__ double cs;
-    if (threadIdx.x == 0) {
-        const double pa = a.p[io];
-        cs = (a.wanted - U[2 * ((size_t)(io + 1) + (size_t)(io + 1) * m)]) / (pa * pa);
-    }
-    __syncthreads();
-    const double cc = cs;
-    for (int e = threadIdx.x; e < n * n; e += kAux) {
-        const int z = e / n, y = e % n;
-        U[2 * ((size_t)(y + 1) + (size_t)(z + 1) * m)] += cc * a.p[y] * a.p[z];
-    }
-}
-
-hipError_t launch_gauge_fix(const GaugeFixArgs& a, const GaugeList& list, hipStream_t st)
-{
-    if (list.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gauge_fix, dim3(list.count), dim3(kAux), 0, st, a, list);
     return hipGetLastError();
 }
 

@@ -63,9 +63,7 @@ struct ProjFoldArgs {
     const double* lam;    // [lam_e | lam_o], NH each, zero padded
     long ncols;           // folded complex columns: nf * nmodes * NH
     int accumulate;       // pass 4: out += U instead of out = U
-    int gauge;            // pass 2: index of the Neumann null mode among the even
-                          // eigenvectors (its ee entry is left 0 on singular planes), -1 if none
-    FieldGroup g;
+    FieldGroup g;         // no singular plane (the folded solves never take one): |den| > 0
 };
 hipError_t launch_proj_fold(int NH, int mode, const ProjFoldArgs& a, hipStream_t st);
 
@@ -86,15 +84,13 @@ __device__ inline double div_eig(double a, double d)
 
 // Passes 2 and 3 of the folded solve (undivided product, eigenvalue divide, product) on
 // LDS-resident quarters, one workgroup per quarter, in place on the planar buffer.  Same
-// results as PROJ_P2P followed by PROJ_P2P_LDIV.  Singular Neumann planes (gauge >= 0 and
-// |xlambda| <= 1e-10) and the Im planes of mode 0 are left untouched.
+// results as PROJ_P2P followed by PROJ_P2P_LDIV.  The Im planes of mode 0 are left untouched.
 struct ProjQuarterArgs {
     const double* fwd;    // folded P^-1 [P_e^-1; P_o^-1], fragment layout
     const double* bwd;    // folded P [P_e; P_o], fragment layout
     const double* lam;    // [lam_e | lam_o], NH each, zero padded
     double* X;            // planar quarters: pass-1 output in, pass-3 output out
     long nplanes;         // real planes: nf * nmodes * 2
-    int gauge;            // index of the Neumann null mode among the even eigenvectors, -1 if none
     FieldGroup g;
 };
 bool proj_quarter_supported(int NH);
@@ -157,20 +153,6 @@ struct GaugeList {
     int plane[kMaxGaugePlanes];  // planar plane indices (2*(f*nmodes+mode)+part)
 };
 hipError_t launch_gauge(const GaugeArgs& a, const GaugeList& list, hipStream_t st);
-
-// Neumann k=0 gauge on folded pass-2 output: valore = sum over the four
-// quarters of s_r s_c prow_r^T G_rc prow_c, then G_ee(io, io) = (wanted - valore) / pdiag^2.
-struct GaugeFoldArgs {
-    double* G;            // folded planar pass-2 output
-    const double* prow;   // [s_e P_e(a', :) | s_o P_o(a', :)], NH each (sign of the mirror folded in)
-    double pdiag;         // P_e(a', io)
-    double wanted;
-    const double* lam;    // [lam_e | lam_o]: G is undivided, terms divided as they are summed
-    int io;               // index of the null mode among the even eigenvectors
-    int NH;
-    FieldGroup g;
-};
-hipError_t launch_gauge_fold(const GaugeFoldArgs& a, const GaugeList& list, hipStream_t st);
 
 // Null-space projection of the residual on the singular (Neumann, xlambda = 0)
 // planes of a refinement step: r -= (q^T r q) p p^T with q the null row of P^-1

@@ -82,7 +82,6 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
     // is divided by lam_rp(k) + lam_cp(l) - xlambda as it is staged in LDS
     int ld_l = 0, ld_cp = 0;
     double ld_xlam = 0.0;
-    bool ld_gauge = false;
 
     const d2v* colA[PPT];                     // C2P: column j' and its mirror n-1-j'
     const d2v* colB[PPT];
@@ -130,7 +129,6 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
                 ld_l = l;
                 ld_cp = cp;
                 ld_xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
-                ld_gauge = a.gauge >= 0 && fabs(ld_xlam) <= 1.0e-10 && cp == 0 && l == a.gauge;
             }
         }
     }
@@ -203,11 +201,11 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
 #pragma unroll
                 for (int e = 0; e < DPT; ++e) {
                     double v = src_ok ? breg[rp][e] : 0.0;
-                    if constexpr (LDIV) {   // solvediag.f90:61-66 / :151-159, the gauge entry kept as set
+                    if constexpr (LDIV) {   // solvediag.f90:61-66
                         const int k = st_k0 + kk0 + e;
                         const int hh = rp ? (n >> 1) : he;
                         const int hc = ld_cp ? (n >> 1) : he;
-                        if (ld_l < hc && k < hh && !(ld_gauge && rp == 0 && k == a.gauge))
+                        if (ld_l < hc && k < hh)
                             v = div_eig(v, lam_s[rp * NH + k] + ld_lamc);
                     }
                     Bs[buf][rp][v_dst * LDB + kk0 + e] = v;
@@ -330,11 +328,9 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
             }
     } else {
         double xlam = 0.0;
-        bool gauge_plane = false;
         if constexpr (MODE == PROJ_P2P_DIV) {
             const int f = (int)((fm) / g.nmodes), mode = (int)(fm % g.nmodes);
             xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
-            gauge_plane = a.gauge >= 0 && fabs(xlam) <= 1.0e-10;   // solvediag.f90:161-206
         }
         const int hh[2] = {he, n >> 1};
 #pragma unroll
@@ -350,8 +346,7 @@ __global__ __launch_bounds__(256, 2) void k_projf(ProjFoldArgs a)
                     const int row = rbase + (mt % TH) * 16 + 4 * r;
                     double v = acc[mt][nt][r];
                     if constexpr (MODE == PROJ_P2P_DIV) {
-                        if (l < hh[cp] && row < hh[rp] &&
-                            !(gauge_plane && cp == 0 && rp == 0 && l == a.gauge && row == a.gauge))
+                        if (l < hh[cp] && row < hh[rp])
                             v = v / (a.lam[cp * NH + l] + a.lam[rp * NH + row] - xlam);
                         else
                             v = 0.0;

@@ -15,10 +15,6 @@
 // (the same MFMA sequence over k, the same divide) is that of k_projf's P2P
 // and P2P_LDIV passes, so the results are identical to the bit.
 //
-// The Neumann gauge (solvediag.f90:161-206) sets one entry of a singular
-// plane's pass-2 output from a sum over all four quarters; those planes are
-// skipped here and go through the separate passes.
-//
 // Work decomposition: 8 waves (two per SIMD), wave w owns output rows
 // 32 (w & 3) .. +32 and columns 64 (w >> 2) .. +64: 2 x 4 tiles of
 // v_mfma_f64_16x16x4f64.  The basis rows come from L2 in the fragment-major
@@ -53,16 +49,14 @@ __global__ __launch_bounds__(kQThreads, 1) void k_projq(ProjQuarterArgs a)
     const int n = g.n;
     const int hh[2] = {(n + 1) >> 1, n >> 1};
 
-    // quarter q = plane * 4 + slot; skipped: Im of mode 0 (int2d3d.f90:43, not used by pass 4) and
-    // the singular planes, whose gauge needs all four quarters
+    // quarter q = plane * 4 + slot; skipped: Im of mode 0 (int2d3d.f90:43, not used by pass 4)
     auto xlam_of = [&](long q) {
         const long fm = q >> 3;
         return xlambda_of(g.alambda[(int)(fm / g.nmodes)], g.alp, g.mode0 + (int)(fm % g.nmodes));
     };
     auto valid = [&](long q) {
         const long fm = q >> 3;
-        const bool im0 = ((q >> 2) & 1) && g.mode0 + (int)(fm % g.nmodes) == 0;
-        return !im0 && !(a.gauge >= 0 && fabs(xlam_of(q)) <= 1.0e-10);
+        return !(((q >> 2) & 1) && g.mode0 + (int)(fm % g.nmodes) == 0);
     };
     auto next_valid = [&](long q) {   // first valid quarter of this workgroup's sequence at or after q
         while (q < nq && !valid(q)) q += gridDim.x;

@@ -5,7 +5,7 @@
 // products' rounding error is what limits the refined solution.  Here both products are
 // formed exactly in integer arithmetic from byte digits:
 //   S(y, k) = 2^e(y) sum_{t=1..7} d_t(y, k) 2^-8t      (host, build_sdig; exact to 2^-57 e)
-//   U(k, z) = 2^g    sum_{u=1..3} c_u(k, z) 2^-8u      (launch_lpq's digit output; exact)
+//   U(k, z) = 2^g    sum_{u=1..kUDig} c_u(k, z) 2^-8u  (launch_lpq's digit output; exact)
 //   (S U)(y, z) = 2^(e(y)+g) sum_w 2^-8w sum_{t+u=w} (d_t c_u)(y, z)
 // with every digit product (d_t c_u) an exact int32 v_mfma_i32_16x16x64_i8 sum over k.  The
 // pairs with t + u > kDigT = 8 are left out: they are below 2^-64 of the row's terms (the
@@ -240,7 +240,6 @@ __global__ __launch_bounds__(256, 1) void k_resid_i8(ResidI8Args a, long ncplx, 
         digit_products<KC, false>(ua, sz, lane, lzv);       // U S_c^T: A = U digits, B = S_c digits
         // r = f - ((S_r U + U S_c^T) - xlambda U), the order of the FP64 residual
         const double xlam = xlambda_of(g.alambda[f], g.alp, g.mode0 + mode);
-        const bool singular = a.neumann && fabs(xlam) <= 1.0e-10;
         double rmx = 0.0, fmx = 0.0;
         float* R = a.R + qp * NH * NH + (size_t)(z0 + l16) * NH + y0 + 4 * l4;
 #pragma unroll
@@ -253,17 +252,18 @@ __global__ __launch_bounds__(256, 1) void k_resid_i8(ResidI8Args a, long ncplx, 
                     const int y = y0 + yt * 16 + 4 * l4 + e, z = z0 + zt * 16 + l16;
                     const double lyv = ldexp(ly[yt][zt][e], ey[yt][e] + gq);
                     const double lz = ldexp(lzv[yt][zt][e], ez[zt] + gq);
-                    // U = 2^(g-24) (c1 2^16 + c2 2^8 + c3), the digits sign-extended from byte e
-                    const int c1 = (int)(cur.uw[yt][zt][0] << (24 - 8 * e)) >> 24;
-                    const int c2 = (int)(cur.uw[yt][zt][1] << (24 - 8 * e)) >> 24;
-                    const int c3 = (int)(cur.uw[yt][zt][2] << (24 - 8 * e)) >> 24;
-                    const double uv = ldexp((double)(c1 * 65536 + c2 * 256 + c3), gq - 24);
+                    // U = 2^(g - 8 kUDig) (c1 256^(kUDig-1) + ... + c_kUDig), the digits
+                    // sign-extended from byte e
+                    int Z = 0;
+#pragma unroll
+                    for (int d = 0; d < kUDig; ++d) Z = Z * 256 + ((int)(cur.uw[yt][zt][d] << (24 - 8 * e)) >> 24);
+                    const double uv = ldexp((double)Z, gq - 8 * kUDig);
                     const double fv = (double)cur.fh[yt][zt][e] + (double)cur.fl[yt][zt][e];
                     double rv = fv - ((lyv + lz) - xlam * uv);
                     const bool in = y < hh[r] && z < hh[c];
                     if (!in) rv = 0.0;
                     if (STATS) {
-                        if (in && !singular) rmx = fmax(rmx, fabs(rv));
+                        if (in) rmx = fmax(rmx, fabs(rv));
                         if (in) fmx = fmax(fmx, fabs(fv));
                     }
                     rout[e] = (float)rv;

@@ -144,15 +144,17 @@ int duct_helm3d_batch_noalloc_dev_(const int *nfields, const duct_devptr_t *rhs,
  * split in 3 terms (6 products), 4 = plain bf16, 5 = the even/odd folded
  * solve: all four passes of a quarter fused in one workgroup, fp16 hi/lo
  * split (3 products), FP32 intermediates, the FP64 residual of the
- * centrosymmetric part of D2B in the folded basis, one extra correction of
- * the Poisson modes with xlambda < 64 (scheme0 == scheme1 == 5, niter <= 2,
+ * centrosymmetric part of D2B in the folded basis; the leading Neumann modes
+ * with xlambda < 64 (the singular k=0 plane among them) are solved by the
+ * unfolded FP64 passes instead (scheme0 == scheme1 == 5, niter <= 2,
  * m with 32 < ceil((m-2)/2) <= 128), 6 = scheme 5 with the residual's two
  * products formed exactly from int8 digits on the int8 matrix cores (seven
  * base-256 digits of each row of the operator, three of the first solve's
  * result rounded to 2^-24 of its quarter's maximum) and summed to FP64
  * accuracy (scheme0 == scheme1 == 6, niter == 1).  niter in [0, 16].
  * stats (host, may be NULL): (niter+1) x nfields x 2 doubles = max|r| after
- * each step (the last one after the final correction) and max|f|.           */
+ * each step (the last one after the final correction) and max|f|, over the
+ * refined modes (schemes 5 and 6: not the FP64-solved low Neumann modes).  */
 int duct_helm3d_mixed_batch_dev_(const int *nfields, const duct_devptr_t *rhs,
                                  const duct_devptr_t *sol, const char *bounds,
                                  const double *alambda, const int *m, const int *nmodes,
@@ -324,10 +326,11 @@ int duct_mat_invdiag_dev_(const int *Nx, const int *Ny, const double *xlam, cons
  * projections.  mode 0 (default): the reference's unfolded products
  * (solvediag.f90:51-80, :140-219); 1: Dirichlet folded, agreeing with the
  * unfolded solve to ~1e-13 relative at m = 257; 2: Neumann folded as well,
- * including the k=0 gauge (:161-206).  The Neumann Poisson planes are
- * ill-conditioned: there the fold moves the result by up to ~4e-11 relative,
- * the size of the unfolded solve's own rounding error on those planes.
- * The reference has no such switch.                                         */
+ * except its leading modes with xlambda < 64 (the singular k=0 plane with its
+ * gauge, :161-206, among them), which take the unfolded passes: on those
+ * ill-conditioned Poisson planes the folded operator's solution differs from
+ * D2B's by up to ~4e-11 of the field at m = 257, from xlambda = 64 up by
+ * ~2e-13.  The reference has no such switch.                                */
 int duct_set_fold_(const int *mode);
 
 /* Concurrency of the boundary groups of a batched solve.  on != 0 (default):
@@ -343,10 +346,8 @@ int duct_set_overlap_(const int *on);
 /* Execution of the folded solve's middle passes (duct_set_fold_ 1 and 2).
  * on != 0 (default): where the folded size is 128 (m = 131 ... 259), the
  * undivided second product, the eigenvalue divide and the third product of
- * each 128 x 128 quarter run in one kernel, with the quarter held in LDS; the
- * singular Neumann planes, whose gauge sums over all four quarters, keep the
- * separate passes.  on = 0: two planar launches with the gauge between them.
- * The results are the same to the bit.  The reference has no such switch
+ * each 128 x 128 quarter run in one kernel, with the quarter held in LDS.
+ * on = 0: two planar launches.  The results are the same to the bit.  The reference has no such switch
  * (its products are cuBLAS calls, solvediag.f90:51-80, :140-219).          */
 int duct_set_fused_quarters_(const int *on);
 
