@@ -141,6 +141,25 @@ def test_fold_neumann(dev, oracle_mod, m, M, ib, alam):
     assert e <= tol
 
 
+def test_fold_neumann_singular_plane_past_mode_0(dev, oracle_mod):
+    """alambda = -100, modes 8 ... 15: xlambda = k^2 - 100 is 0 at k = 10 (a singular plane with
+    its gauge, solvediag.f90:151, past mode 0) and below 64 up to k = 12, so modes 8-12 take the
+    unfolded passes and 13-15 the folded ones; both parts against the oracle at the FP64 bar."""
+    m, M, ib, alam = 129, 8, 8, -100.0
+    ob = ensure(dev, oracle_mod, m)
+    rng = np.random.default_rng(101)
+    f = rng.standard_normal((M, m, m)) + 1j * rng.standard_normal((M, m, m))
+    ref, _ = oracle_mod.helm3d(ob, f, "n", alam, ib=ib)
+    got = solve(dev, f, "n", alam, 2, ib=ib)
+    io = dev.solver._state["setup"].neumann.iofzero
+    k0 = 10 - ib
+    assert abs(got[k0, io, io]) <= 1e-12 * np.abs(got[k0]).max()   # the gauge node holds `wanted` (0)
+    e, e_low, e_fold = relerr(got, ref), relerr(got[:5], ref[:5]), relerr(got[5:], ref[5:])
+    print("singular plane at k=10: rel.err %.2e (unfolded modes %.2e, folded modes %.2e)" % (e, e_low, e_fold),
+          flush=True)
+    assert e <= TOL
+
+
 def test_fold_all_config2_batch(dev, oracle_mod):
     """257^2 x 256 x 4 with fold mode 2 (every field folded but the Poisson modes with xlambda < 64):
     sampled modes on both sides of that switch against the oracle, <= 1e-12 of the largest sampled

@@ -189,6 +189,28 @@ def test_fold16_full_size_config2(dev, oracle_mod, scheme, niter):
     assert np.all(st[-1, :, 0] < 1e-4 * st[0, :, 0])
 
 
+@pytest.mark.parametrize("scheme", ["fold16", "fold16q"])
+def test_fold16_singular_plane_past_mode_0(dev, oracle_mod, scheme):
+    """Neumann with alambda = -100 at modes 8 ... 15: the singular plane (xlambda = 0 at k = 10) and
+    the other modes with xlambda < 64 take the FP64 passes, modes 13-15 are refined; <= 1e-10."""
+    import torch
+    m, M, ib, alam = 129, 8, 8, -100.0
+    ensure(dev, m)
+    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(m))
+    rng = np.random.default_rng(103)
+    f = rng.standard_normal((M, m, m)) + 1j * rng.standard_normal((M, m, m))
+    ref, _ = oracle_mod.helm3d(ob, f, "n", alam, ib=ib)
+    r = torch.from_numpy(np.ascontiguousarray(f)).cuda()
+    u = torch.zeros_like(r)
+    dev.helm3d_mixed_batch([r], [u], "n", [alam], ib=ib, scheme0=scheme, scheme1=scheme, niter=1)
+    torch.cuda.synchronize()
+    got = u.cpu().numpy()
+    e = relerr(got, ref)
+    print("%s, singular plane at k=10: rel.err %.2e (refined modes %.2e)" % (scheme, e, relerr(got[5:], ref[5:])),
+          flush=True)
+    assert e <= TOL
+
+
 def test_fold16q_equals_fold16(dev):
     """Scheme 6 (int8-digit residual) against scheme 5 (FP64-MFMA residual) on every mode of a
     129^2 x 12-mode batch: the two refined solutions agree to 2e-11 of the field maximum (each is
