@@ -907,8 +907,8 @@ int solve_fields_mixed(int nf, const duct_devptr_t* rhs, const duct_devptr_t* so
                         "(m=%d, bound '%c')", m, bidx ? 'n' : 'd');
     }
     // The folded solve with both groups: the Neumann group on the side stream with its own work
-    // arrays, so that its latency-bound kernels (null projection, low-mode correction, gauge,
-    // edges) fill the gaps of the Dirichlet group's (not while launches are timed).
+    // arrays, so that its latency-bound kernels (the low modes' FP64 passes, gauge, edges) fill the
+    // gaps of the Dirichlet group's (not while launches are timed).
     if (folded && gs[0].nf && gs[1].nf && L.overlap && !L.prof) {
         if (!L.side) {
             HIPCHK(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
