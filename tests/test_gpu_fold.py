@@ -160,37 +160,33 @@ def test_fold_neumann_singular_plane_past_mode_0(dev, oracle_mod):
     assert e <= TOL
 
 
-def test_fold_all_config2_batch(dev, oracle_mod):
-    """257^2 x 256 x 4 with fold mode 2 (every field folded but the Poisson modes with xlambda < 64):
-    sampled modes on both sides of that switch against the oracle, <= 1e-12 of the largest sampled
-    mode of each field (modes 0-2 carry the field's maximum)."""
+def test_fold_all_config2_batch(dev, oracle_mod, config2_refs):
+    """257^2 x 256 x 4 with fold mode 2 (every field folded but the Poisson modes with xlambda < 64,
+    which take the unfolded passes): every mode of every field against the oracle at the FP64 bar
+    (max-norm error over the field's maximum); the folded Poisson modes from k = 8 up are also
+    reported against their own maxima."""
     import torch
     from duct_amd import workloads
     cfg = workloads.SOLVE_257
-    ob = ensure(dev, oracle_mod, cfg.m)
-    fields = [workloads.rhs_field(cfg, f) for f in range(4)]
+    ensure(dev, oracle_mod, cfg.m)
+    fields, alams, refs = config2_refs
     rhs = [torch.from_numpy(x).cuda() for x in fields]
     sol = [torch.zeros_like(r) for r in rhs]
-    alams = [workloads.alambda_of(cfg, f) for f in range(4)]
     dev.set_fold(2)
     try:
         dev.helm3d_batch(rhs, sol, workloads.BOUNDS, alams)
         torch.cuda.synchronize()
     finally:
         dev.set_fold(0)
-    oracle_mod.use_blas(True)
-    try:
-        for f in range(4):
-            ks = (0, 1, 2, 7, 8, 9, 40, 255)
-            refs = [oracle_mod.helm3d(ob, fields[f][k:k + 1], workloads.BOUNDS[f], alams[f], ib=k)[0] for k in ks]
-            scale = max(np.max(np.abs(r)) for r in refs)
-            for k0, ref in zip(ks, refs):
-                d = float(np.max(np.abs(sol[f][k0:k0 + 1].cpu().numpy() - ref)))
-                print("257^2x256 fold-all field %s mode %d: error %.2e of the field, %.2e of the mode" % (
-                    workloads.FIELDS[f], k0, d / scale, d / np.max(np.abs(ref))), flush=True)
-                assert d <= TOL * scale
-    finally:
-        oracle_mod.use_blas(False)
+    for f in range(4):
+        got = sol[f].cpu().numpy()
+        e = relerr(got, refs[f])
+        d = np.max(np.abs(got - refs[f]).reshape(got.shape[0], -1), axis=1)
+        own = d / np.maximum(np.max(np.abs(refs[f]).reshape(got.shape[0], -1), axis=1), 1e-300)
+        print("257^2x256 fold-all field %s: %.2e of the field over all modes; worst mode %d; worst mode "
+              "against its own maximum %.2e (k = %d)" % (workloads.FIELDS[f], e, int(np.argmax(d)),
+                                                          float(own.max()), int(np.argmax(own))), flush=True)
+        assert e <= TOL
 
 
 def _rotated_pair_basis(P, Pinv, lam, theta):

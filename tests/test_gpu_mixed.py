@@ -136,54 +136,40 @@ def test_mixed_rejects_bad_scheme(dev):
              byref_int(0), byref_double(1.0), byref_int(9), byref_int(2), byref_int(1), None, ctypes.c_size_t(1))
 
 
-def test_mixed_full_size_config2(dev, oracle_mod):
-    """257^2 x 256 modes x 4 fields, bf16x6 solve + one bf16x6 refinement (the bench default): <= 1e-10 on sampled modes."""
-    import torch
+def _check_all_modes(label, got, refs):
+    """Every mode of every field against the oracle: max-norm error over the field's maximum."""
     from duct_amd import workloads
-    cfg = workloads.SOLVE_257
-    ensure(dev, cfg.m)
-    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(cfg.m))
-    fields = [workloads.rhs_field(cfg, f) for f in range(4)]
-    alams = [workloads.alambda_of(cfg, f) for f in range(4)]
+    for f in range(4):
+        e = relerr(got[f], refs[f])
+        d = np.max(np.abs(got[f] - refs[f]).reshape(got[f].shape[0], -1), axis=1)
+        print("%s 257^2x256 field %s: %.2e of the field over all modes (worst mode %d)" % (
+            label, workloads.FIELDS[f], e, int(np.argmax(d))), flush=True)
+        assert e <= TOL
+
+
+def test_mixed_full_size_config2(dev, config2_refs):
+    """257^2 x 256 modes x 4 fields, bf16x6 solve + one bf16x6 refinement (the bench's unfolded
+    refined path): every mode <= 1e-10 of its field."""
+    from duct_amd import workloads
+    ensure(dev, workloads.SOLVE_257.m)
+    fields, alams, refs = config2_refs
     got, st = run_mixed(dev, fields, workloads.BOUNDS, alams, scheme0="bf16x6", scheme1="bf16x6", niter=1)
-    oracle_mod.use_blas(True)
-    try:
-        for f in range(4):
-            for k0 in (0, 1, 2, 100, 255):
-                ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
-                e = relerr(got[f][k0:k0 + 1], ref)
-                print("257^2x256 field %s mode %d rel.err %.2e" % (workloads.FIELDS[f], k0, e), flush=True)
-                assert e <= TOL
-    finally:
-        oracle_mod.use_blas(False)
+    _check_all_modes("bf16x6 x1", got, refs)
 
 
 @pytest.mark.parametrize("scheme,niter", [("fold16", 1), ("fold16q", 1), ("f32", 1), ("f32", 2)])
-def test_fold16_full_size_config2(dev, oracle_mod, scheme, niter):
+def test_fold16_full_size_config2(dev, config2_refs, scheme, niter):
     """257^2 x 256 modes x 4 fields: the folded fused solve with one refinement (scheme 5: FP64-MFMA
     residual; scheme 6: the residual from exact int8 digit products), and the FP32-MFMA eigenbasis
-    products (scheme 1) with one and two FP64 refinements: <= 1e-10 on sampled modes, on both sides
-    of xlambda = 64 (the folded schemes solve the Poisson modes below it with the FP64 passes)."""
+    products (scheme 1) with one and two FP64 refinements: every mode <= 1e-10 of its field (the
+    folded schemes solve the Poisson modes with xlambda < 64 with the FP64 passes)."""
     from duct_amd import workloads
-    cfg = workloads.SOLVE_257
-    ensure(dev, cfg.m)
-    ob = oracle_mod.OracleBasis(oracle_mod.pseudomat(cfg.m))
-    fields = [workloads.rhs_field(cfg, f) for f in range(4)]
-    alams = [workloads.alambda_of(cfg, f) for f in range(4)]
+    ensure(dev, workloads.SOLVE_257.m)
+    fields, alams, refs = config2_refs
     got, st = run_mixed(dev, fields, workloads.BOUNDS, alams, scheme0=scheme, scheme1=scheme, niter=niter)
     print("%s x%d residual history:" % (scheme, niter), [" ".join("%.1e" % (r / fm) for r, fm in st[:, f]) for f in range(4)],
           flush=True)
-    oracle_mod.use_blas(True)
-    try:
-        for f in range(4):
-            for k0 in (0, 1, 2, 7, 8, 100, 255):
-                ref, _ = oracle_mod.helm3d(ob, fields[f][k0:k0 + 1], workloads.BOUNDS[f], alams[f], ib=k0)
-                e = relerr(got[f][k0:k0 + 1], ref)
-                print("%s x%d 257^2x256 field %s mode %d rel.err %.2e" % (scheme, niter, workloads.FIELDS[f], k0, e),
-                      flush=True)
-                assert e <= TOL
-    finally:
-        oracle_mod.use_blas(False)
+    _check_all_modes("%s x%d" % (scheme, niter), got, refs)
     assert not got[0][0].imag.any()
     # the correction reduces the residual by >= 1e4 on every field
     assert np.all(st[-1, :, 0] < 1e-4 * st[0, :, 0])
