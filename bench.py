@@ -64,7 +64,7 @@ PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "f16": 2.5e15,   # MI35
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters over a plain 1 GiB copy
 # (profiles/k_projf_k_projq_f64_fold_257x256x4_pmc.md, profiles/k_lpq_fold16q_257x256x4_traffic.md:
 # fetch counted 0.5 per byte, write 1.0).
-TRAFFIC = {("fold", 257, 256, 4): 1133.4e6, ("fold_mid", 257, 256, 4): 1117.6e6, ("lp_p2p", 257, 256, 4): 493.0e6}
+TRAFFIC = {("fold", 257, 256, 4): 1124.0e6, ("fold_mid", 257, 256, 4): 1110.2e6, ("lp_p2p", 257, 256, 4): 609.6e6}
 TOL = 1e-10        # the metric's tolerance (refined and folded paths)
 TOL_FP64 = 1e-12   # the FP64 bar (plain FP64 paths)
 
