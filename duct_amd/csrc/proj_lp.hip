@@ -57,8 +57,10 @@ __device__ __forceinline__ void split_terms(double x, u16 (&t)[T])
     }
 }
 
+// Two workgroups per CU, except the 3-term bf16 split at the smallest tiles (NP <= 64: its operand
+// terms do not fit two waves per SIMD).
 template <int NP, int MODE, int S>
-__global__ __launch_bounds__(256, 2) void k_proj_lp(ProjLpArgs a)
+__global__ __launch_bounds__(256, (NP <= 64 && S == LP_BF16X6) ? 1 : 2) void k_proj_lp(ProjLpArgs a)
 {
     using Sh = ProjShape<NP>;
     using Tr = LpT<S>;
