@@ -463,6 +463,9 @@ def main():
                         "max_rel_err_vs_oracle": res[mixed_best].get("max_rel_err_vs_oracle")} if mixed_best else None),
         "mixed_vs_fp64": (res[mixed_best]["mode_solves_per_s"] / res["fp64"]["mode_solves_per_s"]
                           if mixed_best and "fp64" in res else None),
+        # and against the fastest FP64 path at the FP64 bar (the folded solve)
+        "mixed_vs_fp64_tier": (res[mixed_best]["mode_solves_per_s"] / res[fp64_best]["mode_solves_per_s"]
+                               if mixed_best and fp64_best else None),
         # the stage's algorithmic work (8 n^3 per real plane, four unfolded products) over the step time;
         # the folded and refined paths execute other amounts, which the roofline counts
         "algorithmic_tflops": 4 * g / (res[chosen]["ms_per_step"] * 1e-3) / 1e12,
