@@ -453,7 +453,8 @@ def main():
         # what `value` is: the fastest path within the metric's tolerance of the FP64 oracle
         "value_path": chosen,
         "value_tier": res[chosen]["tier"],
-        "value_tolerance": TOL,
+        "value_tolerance": res[chosen]["bar"],   # the bar of the chosen path's tier
+        "selection_tolerance": TOL,              # value: the fastest path within this of the oracle
         "value_max_rel_err_vs_oracle": res[chosen].get("max_rel_err_vs_oracle"),
         "verified_against_oracle": refs is not None,
         # the FP64 tier at the FP64 bar, and the refined solves against the plain FP64 solve
